@@ -1,0 +1,118 @@
+"""ctypes binding of libchimera_hip.so (the C ABI declared in include/chimera_hip.h).
+
+The library is required: importing the GPU entry points without it raises
+immediately (there is no CPU fallback in the product path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libchimera_hip.so"
+
+c_dp = C.POINTER(C.c_double)
+c_u32p = C.POINTER(C.c_uint32)
+
+
+class BC(C.Structure):
+    """chimera_bc_t: per-side flags (1 periodic, 2 fixed, 3 outflow, 4 inflow)."""
+    _fields_ = [("xlo", C.c_int), ("xhi", C.c_int), ("ylo", C.c_int), ("yhi", C.c_int)]
+
+    @classmethod
+    def of(cls, bc):
+        if isinstance(bc, BC):
+            return bc
+        return cls(*[int(v) for v in bc])
+
+
+class Scheme(C.Structure):
+    """chimera_scheme_t (the compile-time choices of src/mhd_config.f90:27-37)."""
+    _fields_ = [("riemann_solver", C.c_int), ("slope_limiter", C.c_int),
+                ("use_slope_limiting", C.c_int), ("fallback_2_mood", C.c_int)]
+
+    @classmethod
+    def of(cls, s):
+        if isinstance(s, Scheme):
+            return s
+        s = list(s)
+        if len(s) == 2:
+            s += [1, 1]
+        if len(s) != 4:
+            raise ValueError("scheme is (solver, limiter[, use_slope_limiting, mood])")
+        return cls(*[int(v) for v in s])
+
+
+class ChimeraError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ChimeraError(
+            f"{LIB_PATH} is missing: build it with `python -m chimera_amd._build` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(str(LIB_PATH))
+    _declare(L)
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().chimera_last_error().decode(errors="replace")
+        raise ChimeraError(f"{what} failed with status {rc}: {msg}")
+
+
+def _declare(L: C.CDLL) -> None:
+    i, d, vp = C.c_int, C.c_double, C.c_void_p
+    pBC, pSch = C.POINTER(BC), C.POINTER(Scheme)
+    L.chimera_last_error.restype = C.c_char_p
+    L.chimera_last_error.argtypes = []
+    sigs = {
+        "chimera_device_count": [C.POINTER(i)],
+        "chimera_fill_ghost_cells": [c_dp, c_dp, c_dp, i, i, pBC],
+        "chimera_compute_gradients": [c_dp, d, i, i, c_dp, c_dp],
+        "chimera_apply_slope_limiter": [c_dp, d, i, i, c_dp, c_dp, i],
+        "chimera_reconstruction": [c_dp, c_dp, c_dp, d, i, i, c_dp, c_dp, c_dp, c_dp, i, pBC, i],
+        "chimera_thermal_pressure_check": [i, i] + [c_dp] * 30,
+        "chimera_compute_fluxes": [c_dp] * 12 + [d, i, i] + [c_dp] * 5 + [i],
+        "chimera_update_conserved": [c_dp, c_dp, c_dp, d, d, i, i, pBC],
+        "chimera_constrained_transport": [c_dp, c_dp, c_dp, c_dp, d, d, i, i, pBC],
+        "chimera_compute_curl_2d": [c_dp, d, i, i, c_dp, c_dp, pBC],
+        "chimera_compute_divB": [c_dp, c_dp, d, i, i, c_dp, pBC],
+        "chimera_average_face_to_cell_B": [c_dp, c_dp, i, i, c_dp, c_dp, pBC],
+        "chimera_get_conserved": [c_dp] * 6 + [d, d, i, i] + [c_dp] * 4,
+        "chimera_get_primitive": [c_dp] * 6 + [d, d, i, i] + [c_dp] * 4,
+        "chimera_apply_conserved_floors": [c_dp] * 6 + [d, d, i, i],
+        "chimera_stage_fluxes": [c_dp, c_dp, c_u32p, d, d, i, i, pBC, pSch],
+        "chimera_dev_stage_fluxes": [vp, vp, vp, d, d, i, i, pBC, pSch, vp],
+        "chimera_dev_compute_fluxes": [vp, vp, d, i, i, vp],
+        "chimera_ctx_create": [C.POINTER(vp), i, i, d, d, pBC, pSch, vp],
+        "chimera_ctx_destroy": [vp],
+        "chimera_ctx_upload": [vp, C.POINTER(vp)],
+        "chimera_ctx_download": [vp, C.POINTER(vp)],
+        "chimera_ctx_steps": [vp, i, i, c_dp, i],
+        "chimera_ctx_get_time": [vp, C.POINTER(d), C.POINTER(i)],
+        "chimera_ctx_set_time": [vp, d, i],
+        "chimera_ctx_last_dt": [vp, C.POINTER(d)],
+        "chimera_ctx_cfl_dt": [vp, i, C.POINTER(d)],
+        "chimera_ctx_divb": [vp, C.POINTER(d), C.POINTER(d)],
+        "chimera_ctx_stage1_mask": [vp, c_u32p],
+        "chimera_ctx_sync": [vp],
+        "chimera_ctx_set_timing": [vp, i],
+        "chimera_ctx_stage_kernel_ms": [vp, C.POINTER(d), C.POINTER(i)],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.chimera_ctx_stream.argtypes = [vp]
+    L.chimera_ctx_stream.restype = vp

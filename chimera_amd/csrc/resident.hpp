@@ -1,0 +1,51 @@
+// Launch helpers shared by the resident single-grid driver, the batched ensemble
+// and the adjoint.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "stage_kernel.hpp"
+
+namespace chimera {
+
+// Stage tile: 32 x 8 cells, one cell per thread in the update phase; LDS holds the
+// primitives of the 36 x 12 halo tile and the 33 x 10 + 34 x 9 face fluxes (46 KiB).
+constexpr int STAGE_TX = 32;
+constexpr int STAGE_TY = 8;
+constexpr int STAGE_THREADS = STAGE_TX * STAGE_TY;
+
+// Per-grid step bookkeeping kept on the device so whole steps can be captured.
+struct StepState {
+    double t;
+    double dt;
+    unsigned long long smax_bits;  // max signal speed, order-preserving bits
+    int out_count;
+    int step;
+};
+
+struct StageLaunch {
+    GridView g;
+    int stage, use_lim, mood, solver, limiter;
+    int tiles_x, ntiles, grid;
+};
+
+struct CflArgs {
+    const double *u[6];
+    int nx, ny, px, py;
+    double vol, gamma;
+    unsigned long long *smax_bits;
+};
+
+int launch_stage(const StageLaunch &L, hipStream_t stream);
+int launch_cfl(const CflArgs &a, hipStream_t stream);
+int launch_dt_finalize(StepState *st, int mode, double dx, const double *dt_seq,
+                       hipStream_t stream);
+int launch_step_end(StepState *st, hipStream_t stream);
+int launch_set_state(StepState *st, double t, int oc, int step, hipStream_t stream);
+int launch_zero_u32(uint32_t *p, size_t n, hipStream_t stream);
+int launch_divb(const double *bx, const double *by, int nx, int ny, int px, int py, double dx,
+                unsigned long long *maxbits, double *sum, hipStream_t stream);
+
+}  // namespace chimera

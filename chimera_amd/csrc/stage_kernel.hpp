@@ -1,0 +1,308 @@
+// Fused SSP-RK3 stage kernel (gfx950): one workgroup advances a TX x TY tile of
+// cells through a whole RK stage in one pass over HBM.
+//
+// Per tile:
+//   phase 1  load the conserved state of the tile plus a 2-cell halo (face B with
+//            one more cell on the lo side), apply the boundary mapping at load time
+//            (periodic wrap / zero-gradient clamp, src/mhd_bc.f90), and store the
+//            stage primitives (src/mhd_field_ops.f90:120-156 face->cell B,
+//            src/mhd_change_states.f90:61-113) in LDS;
+//   phase 2  for every x face (TX+1 x TY+2) and y face (TX+2 x TY+1) the tile needs:
+//            limited slopes (src/mhd_derivatives.f90:50-127), MUSCL face states and
+//            the MOOD bounds check (130-229), the thermal-pressure check (232-293) and
+//            the Riemann flux (src/mhd_flux.f90:139-779); fluxes go to LDS;
+//   phase 3  per cell: flux-difference update (mhd_flux.f90:20-76), conserved floors
+//            with the stage's cell B (change_states:116-166), constrained transport of
+//            the face B (mhd_flux.f90:79-136), the outflow boundary correction and the
+//            RK blend with floors (src/main.f90:107-267); one write of the new state.
+//
+// The blockIdx -> tile map is XCD-aware: the 8 XCDs receive consecutive blocks
+// round-robin, so block b is sent to tile (b % 8) * (grid / 8) + b / 8, giving each
+// XCD a contiguous band of tiles whose halos then hit its own L2.
+#pragma once
+
+#include <cstdint>
+
+#include "mhd_phys.hpp"
+
+namespace chimera {
+
+// One grid as the stage kernels see it.
+struct GridView {
+    const double *uin[6];  // stage input: Mass, Momx, Momy, Energy, b_x, b_y
+    const double *u0[6];   // u^n (read only by stages 2 and 3)
+    double *uout[6];       // stage output
+    const double *dt;      // device scalar
+    uint32_t *mask;        // optional MOOD / thermal-check bits (stage 1 diagnostics)
+    int nx, ny;
+    int px, py;            // periodic x / y
+    double dx, vol, gamma;
+};
+
+template <int TX, int TY>
+struct TileShape {
+    static constexpr int PW = TX + 4, PH = TY + 4;     // primitives with a 2-cell halo
+    static constexpr int FXW = TX + 1, FXH = TY + 2;   // x faces i in [-1,TX-1], j in [-1,TY]
+    static constexpr int FYW = TX + 2, FYH = TY + 1;   // y faces i in [-1,TX],   j in [-1,TY-1]
+    static constexpr int P_SZ = PW * PH;
+    static constexpr int FX_SZ = FXW * FXH;
+    static constexpr int FY_SZ = FYW * FYH;
+    static constexpr int LDS_DOUBLES = 6 * P_SZ + 5 * FX_SZ + 5 * FY_SZ;
+};
+
+__device__ __forceinline__ int map_index(int g, int n, int periodic) {
+    if (periodic) {
+        int r = g % n;
+        return r < 0 ? r + n : r;
+    }
+    return g < 0 ? 0 : (g >= n ? n - 1 : g);
+}
+
+// Field order in LDS / face registers: rho, vx, vy, P, Bx, By.
+enum { F_RHO = 0, F_VX = 1, F_VY = 2, F_P = 3, F_BX = 4, F_BY = 5 };
+
+// Face states of one field at the two faces a thread handles: the MUSCL left state
+// (cell c extrapolated forward), the right state (next cell extrapolated backward,
+// or the cell value itself at a non-periodic hi boundary), then MOOD against the
+// 5-point stencil of cell c and the rho/P floor.  Returns the MOOD bits (1: left
+// reset, 2: right reset).
+template <int LIMITER>
+__device__ __forceinline__ unsigned face_pair(double m, double c, double p, double pp, double nA,
+                                              double nB, bool last, bool use_lim, bool mood,
+                                              bool rho_or_p, double inv_dx, double inv_2dx,
+                                              double hdx, double &fl, double &fr) {
+    double sl, sr;
+    if (use_lim) {
+        sl = limited_slope<LIMITER>(m, c, p, inv_dx);
+        sr = limited_slope<LIMITER>(c, p, pp, inv_dx);
+    } else {
+        sl = central_slope(m, p, inv_2dx);
+        sr = central_slope(c, pp, inv_2dx);
+    }
+    double l = c + hdx * sl;
+    double r = last ? c : p - hdx * sr;
+    unsigned bits = 0;
+    if (mood) {
+        double lo = fmin(fmin(fmin(fmin(c, p), m), nA), nB);
+        double hi = fmax(fmax(fmax(fmax(c, p), m), nA), nB);
+        if (rho_or_p) lo = fmax(1e-9, lo);
+        if (l < lo || l > hi) { l = c; bits |= 1u; }
+        if (r < lo || r > hi) { r = c; bits |= 2u; }
+    }
+    if (rho_or_p) {
+        l = fmax(l, 1.0e-8);
+        r = fmax(r, 1.0e-8);
+    }
+    fl = l;
+    fr = r;
+    return bits;
+}
+
+// Thermal-pressure check of one face (src/mhd_derivatives.f90:262-267): when the
+// implied thermal pressure is below P_floor every field takes the cell value.
+__device__ __forceinline__ bool thermal_reset(double f[6], const double c[6]) {
+    double p_th = f[F_P] - 0.5 * (f[F_BX] * f[F_BX] + f[F_BY] * f[F_BY]);
+    if (p_th < P_FLOOR) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) f[q] = c[q];
+        return true;
+    }
+    return false;
+}
+
+template <int SOLVER, int LIMITER, int TX, int TY>
+__device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool use_lim, bool mood,
+                                           int tile_x, int tile_y, double *lds) {
+    using S = TileShape<TX, TY>;
+    constexpr int NT = TX * TY;
+    const int tid = threadIdx.x;
+    const int nx = g.nx, ny = g.ny, px = g.px, py = g.py;
+    const int x0 = tile_x * TX, y0 = tile_y * TY;
+    double *sP = lds;                       // [6][PH][PW]
+    double *sFX = lds + 6 * S::P_SZ;        // [5][FXH][FXW]
+    double *sFY = sFX + 5 * S::FX_SZ;       // [5][FYH][FYW]
+    const double dx = g.dx, vol = g.vol, gam = g.gamma;
+    const double inv_dx = 1.0 / dx;
+    const double inv_2dx = 1.0 / (2.0 * dx);
+    const double hdx = 0.5 * dx;
+
+    // ---- phase 1: halo load and stage primitives -------------------------------
+    for (int idx = tid; idx < S::P_SZ; idx += NT) {
+        int lx = idx % S::PW, ly = idx / S::PW;
+        int gi = map_index(x0 - 2 + lx, nx, px);
+        int gj = map_index(y0 - 2 + ly, ny, py);
+        int gim = gi > 0 ? gi - 1 : (px ? nx - 1 : 0);
+        int gjm = gj > 0 ? gj - 1 : (py ? ny - 1 : 0);
+        size_t k = (size_t)gi + (size_t)nx * gj;
+        double Bx = 0.5 * (g.uin[4][k] + g.uin[4][(size_t)gim + (size_t)nx * gj]);
+        double By = 0.5 * (g.uin[5][k] + g.uin[5][(size_t)gi + (size_t)nx * gjm]);
+        double rho, vx, vy, P;
+        cons_to_prim(g.uin[0][k], g.uin[1][k], g.uin[2][k], g.uin[3][k], Bx, By, gam, vol, rho, vx,
+                     vy, P);
+        sP[0 * S::P_SZ + idx] = rho;
+        sP[1 * S::P_SZ + idx] = vx;
+        sP[2 * S::P_SZ + idx] = vy;
+        sP[3 * S::P_SZ + idx] = P;
+        sP[4 * S::P_SZ + idx] = Bx;
+        sP[5 * S::P_SZ + idx] = By;
+    }
+    __syncthreads();
+
+    const bool want_mask = g.mask != nullptr;
+
+    // ---- phase 2a: x faces ------------------------------------------------------
+    for (int idx = tid; idx < S::FX_SZ; idx += NT) {
+        int fx = idx % S::FXW, fy = idx / S::FXW;      // face i = fx-1, row j = fy-1
+        int lx = fx + 1, ly = fy + 1;                  // LDS position of the left cell
+        int ci = map_index(x0 + fx - 1, nx, px);
+        bool last = !px && ci == nx - 1;
+        double L[6], R[6], C[6];
+        unsigned bits = 0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double *row = sP + q * S::P_SZ + ly * S::PW;
+            double c = row[lx];
+            C[q] = c;
+            bits |= face_pair<LIMITER>(row[lx - 1], c, row[lx + 1], row[lx + 2], row[lx + S::PW],
+                                       row[lx - S::PW], last, use_lim, mood,
+                                       q == F_RHO || q == F_P, inv_dx, inv_2dx, hdx, L[q], R[q])
+                    << (4 * q);
+        }
+        if (thermal_reset(L, C)) bits |= 1u << 24;
+        if (thermal_reset(R, C)) bits |= 1u << 25;
+        FaceState a{L[F_RHO], L[F_VX], L[F_VY], L[F_P], L[F_BX], L[F_BY]};
+        FaceState b{R[F_RHO], R[F_VX], R[F_VY], R[F_P], R[F_BX], R[F_BY]};
+        Flux5 f = riemann<SOLVER>(a, b, gam);
+        sFX[0 * S::FX_SZ + idx] = f.mass;
+        sFX[1 * S::FX_SZ + idx] = f.mom_n;
+        sFX[2 * S::FX_SZ + idx] = f.mom_t;
+        sFX[3 * S::FX_SZ + idx] = f.energy;
+        sFX[4 * S::FX_SZ + idx] = f.bt;
+        if (want_mask) {
+            int i = fx - 1, j = fy - 1;
+            if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny && bits)
+                atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
+        }
+    }
+
+    // ---- phase 2b: y faces (rotated: normal vy / By) ----------------------------
+    for (int idx = tid; idx < S::FY_SZ; idx += NT) {
+        int fx = idx % S::FYW, fy = idx / S::FYW;      // face column i = fx-1, j = fy-1
+        int lx = fx + 1, ly = fy + 1;                  // LDS position of the lower cell
+        int cj = map_index(y0 + fy - 1, ny, py);
+        bool last = !py && cj == ny - 1;
+        double L[6], R[6], C[6];
+        unsigned bits = 0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double *col = sP + q * S::P_SZ + ly * S::PW + lx;
+            double c = col[0];
+            C[q] = c;
+            bits |= face_pair<LIMITER>(col[-S::PW], c, col[S::PW], col[2 * S::PW], col[1], col[-1],
+                                       last, use_lim, mood, q == F_RHO || q == F_P, inv_dx,
+                                       inv_2dx, hdx, L[q], R[q])
+                    << (4 * q + 2);
+        }
+        if (thermal_reset(L, C)) bits |= 1u << 26;
+        if (thermal_reset(R, C)) bits |= 1u << 27;
+        FaceState a{L[F_RHO], L[F_VY], L[F_VX], L[F_P], L[F_BY], L[F_BX]};
+        FaceState b{R[F_RHO], R[F_VY], R[F_VX], R[F_P], R[F_BY], R[F_BX]};
+        Flux5 f = riemann<SOLVER>(a, b, gam);
+        sFY[0 * S::FY_SZ + idx] = f.mass;
+        sFY[1 * S::FY_SZ + idx] = f.mom_t;   // Momx through y faces
+        sFY[2 * S::FY_SZ + idx] = f.mom_n;   // Momy through y faces
+        sFY[3 * S::FY_SZ + idx] = f.energy;
+        sFY[4 * S::FY_SZ + idx] = f.bt;      // Bx flux through y faces
+        if (want_mask) {
+            int i = fx - 1, j = fy - 1;
+            if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny && bits)
+                atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: update, floors, CT, outflow correction, blend -----------------
+    for (int c = tid; c < TX * TY; c += NT) {
+        int ci = c % TX, cj = c / TX;
+        int gi = x0 + ci, gj = y0 + cj;
+        if (gi >= nx || gj >= ny) continue;
+        size_t k = (size_t)gi + (size_t)nx * gj;
+        const double dt = *g.dt;
+        const double dtdx = dt * dx;
+        // x faces: FX(i, j) at [j+1][i+1]; y faces: FY(i, j) at [j+1][i+1]
+        const int xr = (cj + 1) * S::FXW + (ci + 1);
+        const int yr = (cj + 1) * S::FYW + (ci + 1);
+        const bool xlo_open = (gi == 0) && !px;
+        const bool ylo_open = (gj == 0) && !py;
+        double U[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double v = g.uin[q][k];
+            double fr = sFX[q * S::FX_SZ + xr];
+            if (xlo_open) v = v - dtdx * fr;
+            else v = v - dtdx * fr + dtdx * sFX[q * S::FX_SZ + xr - 1];
+            double ft = sFY[q * S::FY_SZ + yr];
+            if (ylo_open) v = v - dtdx * ft;
+            else v = v - dtdx * ft + dtdx * sFY[q * S::FY_SZ + yr - S::FYW];
+            U[q] = v;
+        }
+        const int pc = (cj + 2) * S::PW + (ci + 2);
+        const double Bxc = sP[4 * S::P_SZ + pc], Byc = sP[5 * S::P_SZ + pc];
+        conserved_floors(U[0], U[1], U[2], U[3], Bxc, Byc, vol, gam);
+
+        // corner EMF: Ez(a,b) = 1/4 (-F_x(By)(a,b) - F_x(By)(a,b+1) + F_y(Bx)(a,b) + F_y(Bx)(a+1,b))
+        auto negEz = [&](int a, int b) -> double {
+            const double *fbx = sFX + 4 * S::FX_SZ;
+            const double *fby = sFY + 4 * S::FY_SZ;
+            double f0 = fbx[(b + 1) * S::FXW + (a + 1)];
+            double up = (!py && y0 + b == ny - 1) ? f0 : fbx[(b + 2) * S::FXW + (a + 1)];
+            double e0 = fby[(b + 1) * S::FYW + (a + 1)];
+            double rt = (!px && x0 + a == nx - 1) ? e0 : fby[(b + 1) * S::FYW + (a + 2)];
+            double Ez = 0.25 * (-f0 - up + e0 + rt);
+            return -Ez;
+        };
+        double dbx, dby;
+        if (ylo_open) dbx = (negEz(ci, cj + 1) - negEz(ci, cj)) * inv_dx;
+        else dbx = (negEz(ci, cj) - negEz(ci, cj - 1)) * inv_dx;
+        if (xlo_open) dby = -((negEz(ci + 1, cj) - negEz(ci, cj)) * inv_dx);
+        else dby = -((negEz(ci, cj) - negEz(ci - 1, cj)) * inv_dx);
+        double bxn = g.uin[4][k] + dt * dbx;
+        double byn = g.uin[5][k] + dt * dby;
+
+        // outflow boundary flux correction (src/main.f90:116-148), stage primitives
+        if (xlo_open || ylo_open) {
+            double rho = sP[0 * S::P_SZ + pc], vx = sP[1 * S::P_SZ + pc];
+            double vy = sP[2 * S::P_SZ + pc], P = sP[3 * S::P_SZ + pc];
+            double h = 0.5 * (Bxc * Bxc + Byc * Byc);
+            double en = (P - h) / (gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy) + h;
+            if (xlo_open) {
+                U[0] = U[0] + dtdx * rho * vx;
+                U[1] = U[1] + dtdx * (rho * (vx * vx) + P - Bxc * Bxc);
+                U[2] = U[2] + dtdx * (rho * vx * vy - Bxc * Byc);
+                U[3] = U[3] + dtdx * ((en + P) * vx - Bxc * (Bxc * vx + Byc * vy));
+            }
+            if (ylo_open) {
+                U[0] = U[0] + dtdx * rho * vy;
+                U[2] = U[2] + dtdx * (rho * (vy * vy) + P - Byc * Byc);
+                U[1] = U[1] + dtdx * (rho * vy * vx - Byc * Bxc);
+                U[3] = U[3] + dtdx * ((en + P) * vy - Byc * (Byc * vy + Bxc * vx));
+            }
+        }
+
+        if (stage > 1) {
+            const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
+            const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) U[q] = c0 * g.u0[q][k] + c1 * U[q];
+            bxn = c0 * g.u0[4][k] + c1 * bxn;
+            byn = c0 * g.u0[5][k] + c1 * byn;
+            conserved_floors(U[0], U[1], U[2], U[3], Bxc, Byc, vol, gam);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) g.uout[q][k] = U[q];
+        g.uout[4][k] = bxn;
+        g.uout[5][k] = byn;
+    }
+}
+
+}  // namespace chimera

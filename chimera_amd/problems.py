@@ -1,0 +1,251 @@
+"""Initial conditions (host side, numpy) for the benchmark and test problems.
+
+These restate the reference's problem setups (src/mhd_init.f90) so that the same
+FP64 input arrays can be fed to the GPU path and to the CPU oracle; they are set
+up once per run and are not part of the per-step hot path.  States are returned
+as a C-contiguous float64 array of shape (6, ny, nx) holding
+(Mass, Momx, Momy, Energy, b_x, b_y): conserved cell integrals and face B.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PI = 3.1415926535897932       # src/mhd_config.f90:54-56
+TWO_PI = 6.2831853071795865
+FOUR_PI = 12.566370614359173
+P_FLOOR = 1.0e-12
+RHO_FLOOR = 1.0e-12
+
+PERIODIC = (1, 1, 1, 1)
+OUTFLOW = (3, 3, 3, 3)
+
+
+@dataclass
+class Problem:
+    """A ready-to-run grid: state (6, ny, nx) plus the parameters the stage needs."""
+    name: str
+    state: np.ndarray
+    dx: float
+    gamma: float
+    bc: tuple
+    scheme: tuple          # (riemann_solver, slope_limiter, use_slope_limiting, mood)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nx(self) -> int:
+        return self.state.shape[2]
+
+    @property
+    def ny(self) -> int:
+        return self.state.shape[1]
+
+
+# ---- field operations used by the setups (src/mhd_field_ops.f90) --------------
+
+def curl_2d(Az: np.ndarray, dx: float, bc) -> tuple[np.ndarray, np.ndarray]:
+    """Backward-difference curl of a corner potential (field_ops:15-57)."""
+    inv_dx = 1.0 / dx
+    bx = np.empty_like(Az)
+    by = np.empty_like(Az)
+    bx[1:, :] = (Az[1:, :] - Az[:-1, :]) * inv_dx
+    bx[0, :] = (Az[0, :] - Az[-1, :]) * inv_dx if bc[2] == 1 else bx[1, :]
+    by[:, 1:] = -((Az[:, 1:] - Az[:, :-1]) * inv_dx)
+    by[:, 0] = -((Az[:, 0] - Az[:, -1]) * inv_dx) if bc[0] == 1 else by[:, 1]
+    return bx, by
+
+
+def face_to_cell_B(bx: np.ndarray, by: np.ndarray, bc) -> tuple[np.ndarray, np.ndarray]:
+    """Cell-centred B from face B (field_ops:120-156)."""
+    Bx = np.empty_like(bx)
+    By = np.empty_like(by)
+    Bx[:, 1:] = 0.5 * (bx[:, 1:] + bx[:, :-1])
+    Bx[:, 0] = 0.5 * (bx[:, 0] + bx[:, -1]) if bc[0] == 1 else bx[:, 0]
+    By[1:, :] = 0.5 * (by[1:, :] + by[:-1, :])
+    By[0, :] = 0.5 * (by[0, :] + by[-1, :]) if bc[2] == 1 else by[0, :]
+    return Bx, By
+
+
+def prim_to_cons(rho, vx, vy, P, Bx, By, gamma, vol):
+    """get_conserved (src/mhd_change_states.f90:16-58)."""
+    r = np.maximum(rho, RHO_FLOOR)
+    p = np.maximum(P, P_FLOOR)
+    M = r * vol
+    half_b2 = 0.5 * (Bx * Bx + By * By)
+    E = ((p - half_b2) / (gamma - 1.0) + 0.5 * r * (vx * vx + vy * vy) + half_b2) * vol
+    return M, M * vx, M * vy, E
+
+
+def _pack(M, Mx, My, E, bx, by) -> np.ndarray:
+    return np.ascontiguousarray(np.stack([M, Mx, My, E, bx, by]).astype(np.float64))
+
+
+def _centres(n: int, dx: float, lo: float = 0.0) -> np.ndarray:
+    # xlin = (i - 0.5) * dx, src/main.f90:36
+    return lo + (np.arange(n) + 0.5) * dx
+
+
+# ---- problem setups ------------------------------------------------------------
+
+def orszag_tang(n: int, scheme=(1, 2, 1, 1)) -> Problem:
+    """Orszag-Tang vortex, src/mhd_init.f90:296-311 (Az at cell centres)."""
+    dx = 1.0 / n
+    x = _centres(n, dx)
+    X, Y = np.meshgrid(x, x, indexing="xy")   # X[j, i] = x_i, Y[j, i] = y_j
+    rho = np.full((n, n), 25.0 / (36.0 * PI))
+    P = np.full((n, n), 5.0 / (12.0 * PI))
+    vx = -np.sin(TWO_PI * Y)
+    vy = np.sin(TWO_PI * X)
+    Az = np.cos(FOUR_PI * X) / (FOUR_PI * np.sqrt(FOUR_PI)) + \
+        np.cos(TWO_PI * Y) / (TWO_PI * np.sqrt(FOUR_PI))
+    bx, by = curl_2d(Az, dx, PERIODIC)
+    Bx, By = face_to_cell_B(bx, by, PERIODIC)
+    P = P + 0.5 * (Bx * Bx + By * By)
+    gamma = 5.0 / 3.0
+    st = _pack(*prim_to_cons(rho, vx, vy, P, Bx, By, gamma, dx * dx), bx, by)
+    return Problem("orszag_tang", st, dx, gamma, PERIODIC, tuple(scheme))
+
+
+def rotor(n: int, scheme=(2, 1, 1, 1)) -> Problem:
+    """MHD rotor, src/mhd_init.f90:399-450 (outflow on all sides, gamma = 1.4)."""
+    dx = 1.0 / n
+    x = _centres(n, dx)
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    rho = np.ones((n, n))
+    vx = np.zeros((n, n))
+    vy = np.zeros((n, n))
+    P = np.ones((n, n))
+    Bx = np.full((n, n), 5.0 / np.sqrt(4.0 * PI))
+    By = np.zeros((n, n))
+    r0, r1, u0 = 0.1, 0.115, 2.0
+    rx, ry = X - 0.5, Y - 0.5
+    r = np.sqrt(rx * rx + ry * ry)
+    core = r <= r0
+    rho[core] = 10.0
+    vx[core] = -u0 * (ry[core] / r0)
+    vy[core] = u0 * (rx[core] / r0)
+    tap = (~core) & (r < r1)
+    f = (r1 - r[tap]) / (r1 - r0)
+    rho[tap] = 1.0 + 9.0 * f
+    vx[tap] = -f * u0 * (ry[tap] / r[tap])
+    vy[tap] = f * u0 * (rx[tap] / r[tap])
+    P = P + 0.5 * (Bx * Bx + By * By)
+    gamma = 1.4
+    st = _pack(*prim_to_cons(rho, vx, vy, P, Bx, By, gamma, dx * dx), Bx.copy(), np.zeros((n, n)))
+    return Problem("rotor", st, dx, gamma, OUTFLOW, tuple(scheme))
+
+
+def kelvin_helmholtz(n: int, scheme=(1, 2, 1, 1)) -> Problem:
+    """Kelvin-Helmholtz shear layer, src/mhd_init.f90:319-352 (B = 0, gamma = 1.4)."""
+    dx = 1.0 / n
+    x = _centres(n, dx)
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    inner = np.abs(Y - 0.5) <= 0.25
+    rho = np.where(inner, 2.0, 1.0)
+    vx = np.where(inner, 0.5, -0.5)
+    sigma, x0, y0, amp = 0.05, 0.5, 0.25, 0.5
+    vy = amp * np.exp(-((X - x0) ** 2 + (Y - y0) ** 2) / (2.0 * sigma ** 2))
+    P = np.full((n, n), 2.5)
+    z = np.zeros((n, n))
+    gamma = 1.4
+    st = _pack(*prim_to_cons(rho, vx, vy, P, z, z, gamma, dx * dx), z.copy(), z.copy())
+    return Problem("kelvin_helmholtz", st, dx, gamma, PERIODIC, tuple(scheme))
+
+
+def field_loop(nx: int = 512, ny: int = 256, Az_scale=None, scheme=(2, 2, 1, 1),
+               return_az: bool = False):
+    """Field-loop advection on [-1,1] x [-0.5,0.5] (dx = 2/nx), v = (2, 1), rho = p = 1,
+    Az = 1e-3 (0.3 - r) inside r < 0.3 sampled at cell centres (as
+    src/mhd_init.f90:372-386 samples it), gamma = 5/3.
+
+    ``Az_scale`` (optional array (ny, nx)) multiplies Az, for perturbed targets."""
+    dx = 2.0 / nx
+    x = _centres(nx, dx, -1.0)
+    y = _centres(ny, dx, -0.5)
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    r = np.sqrt(X * X + Y * Y)
+    Az = np.where(r < 0.3, 1.0e-3 * (0.3 - r), 0.0)
+    if Az_scale is not None:
+        Az = Az * Az_scale
+    prob = field_loop_from_az(Az, dx, scheme)
+    return (prob, Az) if return_az else prob
+
+
+def field_loop_from_az(Az: np.ndarray, dx: float, scheme=(2, 2, 1, 1)) -> Problem:
+    ny, nx = Az.shape
+    bx, by = curl_2d(Az, dx, PERIODIC)
+    Bx, By = face_to_cell_B(bx, by, PERIODIC)
+    rho = np.ones((ny, nx))
+    vx = np.full((ny, nx), 2.0)
+    vy = np.full((ny, nx), 1.0)
+    P = 1.0 + 0.5 * (Bx * Bx + By * By)
+    gamma = 5.0 / 3.0
+    st = _pack(*prim_to_cons(rho, vx, vy, P, Bx, By, gamma, dx * dx), bx, by)
+    return Problem("field_loop", st, dx, gamma, PERIODIC, tuple(scheme))
+
+
+# ---- Gaussian-random-field ensemble -------------------------------------------
+
+def rbf_field(n: int, ell: float, rng: np.random.Generator) -> np.ndarray:
+    """White noise convolved (periodically, by FFT) with exp(-r^2 / (2 ell^2)) on the
+    unit torus, normalised to unit rms."""
+    w = rng.standard_normal((n, n))
+    k = np.fft.fftfreq(n, d=1.0 / n) * TWO_PI
+    KX, KY = np.meshgrid(k, k, indexing="xy")
+    # Fourier transform of the periodic Gaussian kernel (up to a constant)
+    S = np.exp(-0.5 * ell * ell * (KX * KX + KY * KY))
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * S))
+    f -= f.mean()
+    rms = np.sqrt(np.mean(f * f))
+    return f / rms if rms > 0 else f
+
+
+def grf_member(n: int, seed: int, scheme=(1, 2, 1, 1), gamma_range=(1.4, 5.0 / 3.0),
+               v0_range=(0.1, 1.0), beta_range=(0.1, 10.0), ell_range=(0.05, 0.15),
+               cf_cap: float = 3.0) -> Problem:
+    """One GRF ensemble member on the periodic unit square.
+
+    psi and Az are RBF-smoothed white noise (length ell ~ U[ell_range]); v is the
+    centred curl of psi (as src/mhd_init.f90:764-773), face B the backward curl of Az
+    (compute_curl_2d), rho = p = 1, gamma ~ U[gamma_range], rms |v| ~ U[v0_range],
+    beta ~ U[beta_range] fixes B_rms = sqrt(2 p / beta), and v alone is scaled down so
+    max(c_f + |v|) <= cf_cap (src/mhd_init.f90:815-831)."""
+    rng = np.random.default_rng(seed)
+    gamma = rng.uniform(*gamma_range)
+    v0 = rng.uniform(*v0_range)
+    beta = rng.uniform(*beta_range)
+    ell = rng.uniform(*ell_range)
+    dx = 1.0 / n
+    psi = rbf_field(n, ell, rng)
+    Az = rbf_field(n, ell, rng)
+    vx = (np.roll(psi, -1, axis=0) - np.roll(psi, 1, axis=0)) / (2.0 * dx)
+    vy = -(np.roll(psi, -1, axis=1) - np.roll(psi, 1, axis=1)) / (2.0 * dx)
+    v_rms = np.sqrt(np.mean(vx * vx + vy * vy))
+    vx *= v0 / v_rms
+    vy *= v0 / v_rms
+    bx, by = curl_2d(Az, dx, PERIODIC)
+    Bx, By = face_to_cell_B(bx, by, PERIODIC)
+    b_rms = np.sqrt(np.mean(Bx * Bx + By * By))
+    b0 = np.sqrt(2.0 * 1.0 / beta)
+    s = b0 / b_rms
+    bx, by, Bx, By = bx * s, by * s, Bx * s, By * s
+    rho = np.ones((n, n))
+    p = np.ones((n, n))
+    cf = np.sqrt(gamma * p / rho + (Bx * Bx + By * By) / rho)
+    vmag = np.sqrt(vx * vx + vy * vy)
+    tmax = float(np.max(cf + vmag))
+    if tmax > cf_cap:
+        sc = cf_cap / tmax
+        vx *= sc
+        vy *= sc
+    P = p + 0.5 * (Bx * Bx + By * By)
+    st = _pack(*prim_to_cons(rho, vx, vy, P, Bx, By, gamma, dx * dx), bx, by)
+    return Problem(f"grf_{n}_{seed}", st, dx, gamma, PERIODIC, tuple(scheme),
+                   {"seed": seed, "ell": ell, "v0": v0, "beta": beta})
+
+
+def grf_sizes(members: int, choices=(128, 256, 384, 512), seed: int = 20261203) -> list[int]:
+    """Grid size of each ensemble member, drawn uniformly from ``choices``."""
+    rng = np.random.default_rng(seed)
+    return [int(c) for c in rng.choice(choices, size=members)]

@@ -1,0 +1,206 @@
+/*
+ * chimera_hip.h -- C ABI of libchimera_hip.so, the MI355X (gfx950) implementation of
+ * CHIMERA's SSP-RK3 stage update (tjnb0/CHIMERA).
+ *
+ * Arrays are FP64 in the reference's Fortran (nx, ny) column-major layout, which is
+ * C [ny][nx]: element (i, j) at i + nx*j.  Boundary flags use the reference's
+ * constants (src/mhd_config.f90:40-43): 1 periodic, 2 fixed, 3 outflow, 4 inflow;
+ * the main loop never supplies an ambient state, so 2/3/4 all act as zero-gradient
+ * (src/mhd_bc.f90:128-139, src/main.f90:349-354).  Every function returns 0 on
+ * success and a negative chimera_status on failure; chimera_last_error() describes
+ * the last failure on the calling thread.
+ *
+ * Three layers:
+ *   1. Drop-ins: one function per reference routine, same argument meaning and order
+ *      as the Fortran, plus the BC flags / scheme choices the Fortran reads from
+ *      module mhd_config.  Host pointers; data is copied to the GPU and back per
+ *      call.  (The *_dev variants take device pointers and a HIP stream.)
+ *   2. Resident single-grid API (chimera_ctx_*): state lives in HBM, whole SSP-RK3
+ *      steps run as fused LDS-tiled stage kernels, dt is reduced on the device.
+ *   3. Batched ensemble (chimera_batch_*) and adjoint (chimera_adj_*).
+ */
+#ifndef CHIMERA_HIP_H
+#define CHIMERA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum chimera_status {
+    CHIMERA_OK = 0,
+    CHIMERA_EINVAL = -1,      /* bad argument (sizes, flags, null pointer) */
+    CHIMERA_EHIP = -2,        /* HIP runtime error */
+    CHIMERA_ENOMEM = -3,      /* device allocation failed */
+    CHIMERA_EUNSUPPORTED = -4 /* valid in the reference, not supported here */
+} chimera_status;
+
+enum { CHIMERA_BC_PERIODIC = 1, CHIMERA_BC_FIXED = 2, CHIMERA_BC_OUTFLOW = 3, CHIMERA_BC_INFLOW = 4 };
+enum { CHIMERA_LIMITER_VAN_LEER = 1, CHIMERA_LIMITER_MC = 2 };
+enum { CHIMERA_RUSANOV = 1, CHIMERA_HLLE = 2, CHIMERA_HLLD = 3 };
+/* dt policies of the resident loop (src/main.f90:74-83) */
+enum {
+    CHIMERA_DT_REFERENCE = 0,  /* 0.3*min(dx/(cf+|v|)), clamp to next output time, 1e-12 floor */
+    CHIMERA_DT_CFL = 1,        /* as 0 without the output-time clamp */
+    CHIMERA_DT_PRESCRIBED = 2  /* dt sequence supplied by the caller */
+};
+
+/* Per-side boundary flags: BC_xlo, BC_xhi, BC_ylo, BC_yhi of src/mhd_config.f90:48-49. */
+typedef struct chimera_bc {
+    int xlo, xhi, ylo, yhi;
+} chimera_bc_t;
+
+/* Scheme choices that are compile-time parameters in src/mhd_config.f90:27-37. */
+typedef struct chimera_scheme {
+    int riemann_solver;     /* CHIMERA_RUSANOV / _HLLE / _HLLD */
+    int slope_limiter;      /* CHIMERA_LIMITER_VAN_LEER / _MC */
+    int use_slope_limiting; /* 0: central gradients (compute_gradients) */
+    int fallback_2_mood;    /* 0: no MOOD bounds check */
+} chimera_scheme_t;
+
+const char *chimera_last_error(void);
+int chimera_device_count(int *count);
+
+/* ------------------------------------------------------------------------------
+ * 1. Drop-ins (host pointers).  Each replaces the Fortran routine named.
+ * ---------------------------------------------------------------------------- */
+
+/* mhd_bc::fill_ghost_cells, src/mhd_bc.f90:105-190.  f_pad is (nx+2) x (ny+2);
+ * f_ambient may be NULL (zero-gradient for fixed/inflow, as main.f90 uses it).
+ * Unlike the reference this also accepts nx != ny. */
+int chimera_fill_ghost_cells(const double *f, double *f_pad, const double *f_ambient, int nx,
+                             int ny, const chimera_bc_t *bc);
+
+/* mhd_derivatives::compute_gradients, src/mhd_derivatives.f90:15-47 */
+int chimera_compute_gradients(const double *f_pad, double dx, int nx, int ny, double *f_dx,
+                              double *f_dy);
+
+/* mhd_derivatives::apply_slope_limiter, src/mhd_derivatives.f90:50-127 */
+int chimera_apply_slope_limiter(const double *f_pad, double dx, int nx, int ny, double *f_dx,
+                                double *f_dy, int slope_limiter);
+
+/* mhd_derivatives::reconstruction, src/mhd_derivatives.f90:130-229 */
+int chimera_reconstruction(const double *f, const double *f_dx, const double *f_dy, double dx,
+                           int nx, int ny, double *f_XL, double *f_XR, double *f_YL,
+                           double *f_YR, int rho_or_p, const chimera_bc_t *bc,
+                           int fallback_2_mood);
+
+/* mhd_derivatives::thermal_pressure_check, src/mhd_derivatives.f90:232-293 */
+int chimera_thermal_pressure_check(int nx, int ny, double *P_XL, double *P_XR, double *P_YL,
+                                   double *P_YR, double *Bx_XL, double *Bx_XR, double *Bx_YL,
+                                   double *Bx_YR, double *By_XL, double *By_XR, double *By_YL,
+                                   double *By_YR, double *rho_XL, double *rho_XR,
+                                   double *rho_YL, double *rho_YR, double *vx_XL,
+                                   double *vx_XR, double *vx_YL, double *vx_YR, double *vy_XL,
+                                   double *vy_XR, double *vy_YL, double *vy_YR,
+                                   const double *rho_0, const double *vx_0, const double *vy_0,
+                                   const double *P_0, const double *Bx_0, const double *By_0);
+
+/* mhd_flux::compute_fluxes, src/mhd_flux.f90:139-191 (rusanov 197-288, hlle 291-405,
+ * hlld 408-779). */
+int chimera_compute_fluxes(const double *rho_L, const double *rho_R, const double *vx_L,
+                           const double *vx_R, const double *vy_L, const double *vy_R,
+                           const double *P_L, const double *P_R, const double *Bx_L,
+                           const double *Bx_R, const double *By_L, const double *By_R,
+                           double gamma, int nx, int ny, double *flux_Mass, double *flux_Momx,
+                           double *flux_Momy, double *flux_Energy, double *flux_By,
+                           int riemann_solver);
+
+/* mhd_flux::update_conserved, src/mhd_flux.f90:20-76 */
+int chimera_update_conserved(double *F, const double *flux_F_X, const double *flux_F_Y,
+                             double dx, double dt, int nx, int ny, const chimera_bc_t *bc);
+
+/* mhd_flux::constrained_transport, src/mhd_flux.f90:79-136 */
+int chimera_constrained_transport(double *bx, double *by, const double *flux_By_X,
+                                  const double *flux_Bx_Y, double dx, double dt, int nx, int ny,
+                                  const chimera_bc_t *bc);
+
+/* mhd_field_ops::compute_curl_2d, src/mhd_field_ops.f90:15-57 */
+int chimera_compute_curl_2d(const double *Az, double dx, int nx, int ny, double *bx,
+                            double *by, const chimera_bc_t *bc);
+
+/* mhd_field_ops::compute_divB, src/mhd_field_ops.f90:60-117 */
+int chimera_compute_divB(const double *bx, const double *by, double dx, int nx, int ny,
+                         double *divB, const chimera_bc_t *bc);
+
+/* mhd_field_ops::average_face_to_cell_B, src/mhd_field_ops.f90:120-156 */
+int chimera_average_face_to_cell_B(const double *b_x, const double *b_y, int nx, int ny,
+                                   double *Bx, double *By, const chimera_bc_t *bc);
+
+/* mhd_change_states::get_conserved, src/mhd_change_states.f90:16-58 */
+int chimera_get_conserved(const double *rho, const double *vx, const double *vy,
+                          const double *P, const double *Bx, const double *By, double gamma,
+                          double vol, int nx, int ny, double *Mass, double *Momx,
+                          double *Momy, double *Energy);
+
+/* mhd_change_states::get_primitive, src/mhd_change_states.f90:61-113 */
+int chimera_get_primitive(const double *Mass, const double *Momx, const double *Momy,
+                          const double *Energy, const double *Bx, const double *By,
+                          double gamma, double vol, int nx, int ny, double *rho, double *vx,
+                          double *vy, double *P);
+
+/* mhd_change_states::apply_conserved_floors, src/mhd_change_states.f90:116-166 */
+int chimera_apply_conserved_floors(double *Mass, double *Momx, double *Momy, double *Energy,
+                                   const double *Bx, const double *By, double vol,
+                                   double gamma, int nx, int ny);
+
+/* main.f90:346-403 compute_stage_fluxes from the stage primitives
+ * prims = {rho, vx, vy, P, Bx, By} (6 arrays of nx*ny, consecutive) into
+ * flux = {Mass_X, Mass_Y, Momx_X, Momx_Y, Momy_X, Momy_Y, Energy_X, Energy_Y, By_X,
+ * Bx_Y} (10 consecutive arrays).  mask (optional) gets the MOOD / thermal-check bits
+ * (bit 4*f+s: MOOD reset face s in {XL,XR,YL,YR} of field f in {rho,vx,vy,P,Bx,By};
+ * bit 24+s: thermal check reset face s). */
+int chimera_stage_fluxes(const double *prims, double *flux, uint32_t *mask, double dx,
+                         double gamma, int nx, int ny, const chimera_bc_t *bc,
+                         const chimera_scheme_t *scheme);
+
+/* Device-pointer forms of the drop-ins used by the fused path's tests and the
+ * Python binding (same semantics, no copies, asynchronous on `stream`). */
+int chimera_dev_stage_fluxes(const double *prims, double *flux, uint32_t *mask, double dx,
+                             double gamma, int nx, int ny, const chimera_bc_t *bc,
+                             const chimera_scheme_t *scheme, void *stream);
+int chimera_dev_compute_fluxes(const double *in12, double *out5, double gamma, int n,
+                               int riemann_solver, void *stream);
+
+/* ------------------------------------------------------------------------------
+ * 2. Resident single-grid API: the main.f90:65-303 loop with state in HBM.
+ * ---------------------------------------------------------------------------- */
+typedef struct chimera_ctx chimera_ctx_t;
+
+/* Allocates three state copies (6 FP64 fields each) on the current device.
+ * stream may be NULL (the context creates its own). */
+int chimera_ctx_create(chimera_ctx_t **ctx, int nx, int ny, double dx, double gamma,
+                       const chimera_bc_t *bc, const chimera_scheme_t *scheme, void *stream);
+int chimera_ctx_destroy(chimera_ctx_t *ctx);
+/* state = {Mass, Momx, Momy, Energy, b_x, b_y}; host or device pointers. */
+int chimera_ctx_upload(chimera_ctx_t *ctx, const double *const *state);
+int chimera_ctx_download(chimera_ctx_t *ctx, double *const *state);
+/* Run nsteps SSP-RK3 steps.  dt_in: nsteps values (host memory) for
+ * CHIMERA_DT_PRESCRIBED, else NULL.  use_graph: replay a captured hipGraph per step. */
+int chimera_ctx_steps(chimera_ctx_t *ctx, int nsteps, int dt_mode, const double *dt_in,
+                      int use_graph);
+/* Simulation time and the reference's outputCount (main.f90:61-62,279-292). */
+int chimera_ctx_get_time(chimera_ctx_t *ctx, double *t, int *out_count);
+int chimera_ctx_set_time(chimera_ctx_t *ctx, double t, int out_count);
+/* dt of the last step taken (device value copied back). */
+int chimera_ctx_last_dt(chimera_ctx_t *ctx, double *dt);
+/* CFL dt of the current state under dt_mode (no step taken). */
+int chimera_ctx_cfl_dt(chimera_ctx_t *ctx, int dt_mode, double *dt);
+/* max|divB| and sum|divB| of the current state (compute_divB, field_ops:60-117). */
+int chimera_ctx_divb(chimera_ctx_t *ctx, double *max_abs, double *sum_abs);
+/* MOOD / thermal masks of stage 1 of the next step (layout as chimera_stage_fluxes). */
+int chimera_ctx_stage1_mask(chimera_ctx_t *ctx, uint32_t *mask_host);
+/* Synchronise the context's stream. */
+int chimera_ctx_sync(chimera_ctx_t *ctx);
+/* The stream the context launches on (hipStream_t). */
+void *chimera_ctx_stream(chimera_ctx_t *ctx);
+/* Average duration (ms) of the fused stage kernel over the last chimera_ctx_steps
+ * call, measured with HIP events on the context's stream (0 if not measured). */
+int chimera_ctx_set_timing(chimera_ctx_t *ctx, int enable);
+int chimera_ctx_stage_kernel_ms(chimera_ctx_t *ctx, double *ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

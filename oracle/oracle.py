@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 from pathlib import Path
 
 import numpy as np
@@ -202,6 +203,30 @@ class COracle:
                 "out_count": oc.value, "mask": mask}
 
 
+def on_big_stack(fn, *args, stack_bytes: int = 4 << 30, **kw):
+    """Run fn on a thread with a large stack.  The reference keeps its N x N work arrays
+    as automatic (stack) arrays, which overflow the default 8 MiB stack beyond ~512^2."""
+    out, err = {}, {}
+
+    def body():
+        try:
+            out["v"] = fn(*args, **kw)
+        except BaseException as e:  # re-raised in the caller
+            err["e"] = e
+
+    old = threading.stack_size()
+    threading.stack_size(stack_bytes)
+    try:
+        th = threading.Thread(target=body)
+        th.start()
+        th.join()
+    finally:
+        threading.stack_size(old)
+    if "e" in err:
+        raise err["e"]
+    return out.get("v")
+
+
 def ref_available() -> bool:
     return all((REF_DIR / f"libchimera_ref_s{s}_l{l}.so").exists()
                for s in (1, 2, 3) for l in (1, 2))
@@ -359,8 +384,8 @@ class RefVariant:
         dti = (np.ascontiguousarray(dt_in, dtype=np.float64) if dt_in is not None
                else np.zeros(1))
         t_c, oc = C.c_double(t), C.c_int(out_count)
-        taken = self.lib.ref_run(_d(state), C.c_double(dx), C.c_double(gamma), nsteps, dt_mode,
-                                 _d(dti), _d(dt_out), C.byref(t_c), C.byref(oc),
-                                 C.c_double(t_end), _d(divb))
+        taken = on_big_stack(self.lib.ref_run, _d(state), C.c_double(dx), C.c_double(gamma),
+                             nsteps, dt_mode, _d(dti), _d(dt_out), C.byref(t_c), C.byref(oc),
+                             C.c_double(t_end), _d(divb))
         return {"steps": taken, "dt": dt_out[:taken], "divb": divb[:taken], "t": t_c.value,
                 "out_count": oc.value}

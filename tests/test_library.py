@@ -1,0 +1,83 @@
+"""CPU checks of the C ABI: the library loads, exports every function declared in
+include/*.h, and rejects bad arguments before touching a GPU."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADERS = sorted((ROOT / "include").glob("*.h"))
+
+
+def declared_functions() -> set[str]:
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(chimera_\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+@pytest.fixture(scope="module")
+def L():
+    from chimera_amd import _lib
+    return _lib.lib()
+
+
+def test_headers_declare_api():
+    names = declared_functions()
+    assert {"chimera_compute_fluxes", "chimera_ctx_create", "chimera_ctx_steps"} <= names
+
+
+def test_library_exports_every_declared_symbol(L):
+    lib = ROOT / "chimera_amd" / "libchimera_hip.so"
+    out = subprocess.run(["nm", "-D", "--defined-only", str(lib)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = declared_functions() - exported
+    assert not missing, f"declared but not exported: {sorted(missing)}"
+    for name in declared_functions():
+        getattr(L, name)          # resolves through dlsym
+
+
+def test_invalid_arguments_rejected_on_host(L):
+    from chimera_amd._lib import BC, Scheme
+    h = C.c_void_p()
+    # grid too small
+    rc = L.chimera_ctx_create(C.byref(h), 2, 8, 0.1, 1.4, C.byref(BC(1, 1, 1, 1)),
+                              C.byref(Scheme(1, 2, 1, 1)), None)
+    assert rc == -1 and b"at least 4" in L.chimera_last_error()
+    # bad solver / limiter / flags
+    assert L.chimera_ctx_create(C.byref(h), 8, 8, 0.1, 1.4, C.byref(BC(1, 1, 1, 1)),
+                                C.byref(Scheme(4, 2, 1, 1)), None) == -1
+    assert L.chimera_ctx_create(C.byref(h), 8, 8, 0.1, 1.4, C.byref(BC(1, 1, 1, 1)),
+                                C.byref(Scheme(1, 3, 1, 1)), None) == -1
+    assert L.chimera_ctx_create(C.byref(h), 8, 8, 0.1, 1.4, C.byref(BC(0, 1, 1, 1)),
+                                C.byref(Scheme(1, 2, 1, 1)), None) == -1
+    # periodic on one side of an axis only
+    assert L.chimera_ctx_create(C.byref(h), 8, 8, 0.1, 1.4, C.byref(BC(1, 3, 1, 1)),
+                                C.byref(Scheme(1, 2, 1, 1)), None) == -4
+    # non-physical gamma / dx
+    assert L.chimera_ctx_create(C.byref(h), 8, 8, 0.1, 1.0, C.byref(BC(1, 1, 1, 1)),
+                                C.byref(Scheme(1, 2, 1, 1)), None) == -1
+    # drop-ins validate before allocating
+    z = np.zeros(16)
+    p = z.ctypes.data_as(C.POINTER(C.c_double))
+    assert L.chimera_compute_fluxes(*([p] * 12), C.c_double(1.4), 4, 4, *([p] * 5), 7) == -1
+    assert L.chimera_apply_slope_limiter(p, C.c_double(0.1), 4, 4, p, p, 0) == -1
+    assert L.chimera_update_conserved(p, p, p, C.c_double(0.1), C.c_double(0.1), 1, 4,
+                                      C.byref(BC(1, 1, 1, 1))) == -1
+
+
+def test_scheme_and_bc_structs():
+    from chimera_amd._lib import BC, Scheme
+    assert C.sizeof(BC) == 16 and C.sizeof(Scheme) == 16
+    s = Scheme.of((3, 1))
+    assert (s.riemann_solver, s.slope_limiter, s.use_slope_limiting, s.fallback_2_mood) == (3, 1, 1, 1)
+    with pytest.raises(ValueError):
+        Scheme.of((1, 2, 3))
