@@ -109,10 +109,28 @@ def _declare(L: C.CDLL) -> None:
         "chimera_ctx_sync": [vp],
         "chimera_ctx_set_timing": [vp, i],
         "chimera_ctx_stage_kernel_ms": [vp, C.POINTER(d), C.POINTER(i)],
+        "chimera_ctx_set_tile_shape": [vp, i],
+        "chimera_ctx_get_tile_shape": [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)],
+        "chimera_ctx_autotune": [vp, i, C.POINTER(i), c_dp],
+        "chimera_batch_create": [C.POINTER(vp), i, C.POINTER(i), C.POINTER(i), c_dp, c_dp, pBC,
+                                 pSch, vp],
+        "chimera_batch_destroy": [vp],
+        "chimera_batch_upload": [vp, i, C.POINTER(vp)],
+        "chimera_batch_download": [vp, i, C.POINTER(vp)],
+        "chimera_batch_steps": [vp, i, i, i],
+        "chimera_batch_member_time": [vp, i, C.POINTER(d), C.POINTER(i), C.POINTER(d)],
+        "chimera_batch_sync": [vp],
+        "chimera_batch_cells": [vp, C.POINTER(C.c_longlong)],
+        "chimera_batch_set_tiling": [vp, i, i],
+        "chimera_batch_get_tiling": [vp, C.POINTER(i), C.POINTER(i)],
+        "chimera_batch_autotune": [vp, i, C.POINTER(i), C.POINTER(i)],
+        "chimera_batch_set_timing": [vp, i],
+        "chimera_batch_stage_kernel_ms": [vp, C.POINTER(d), C.POINTER(i)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = C.c_int
-    L.chimera_ctx_stream.argtypes = [vp]
-    L.chimera_ctx_stream.restype = vp
+    for name in ("chimera_ctx_stream", "chimera_batch_stream"):
+        getattr(L, name).argtypes = [vp]
+        getattr(L, name).restype = vp

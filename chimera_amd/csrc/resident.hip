@@ -45,26 +45,31 @@ int check_scheme(const chimera_scheme_t *s) {
 // ---------------------------------------------------------------------------------
 // kernels
 
-template <int SOLVER, int LIMITER>
+template <int SOLVER, int LIMITER, int TX, int TY>
 __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(GridView g, int stage, int use_lim,
                                                              int mood, int tiles_x, int ntiles) {
-    __shared__ double lds[TileShape<STAGE_TX, STAGE_TY>::LDS_DOUBLES];
+    static_assert(TX * TY == STAGE_THREADS, "one cell per thread");
+    __shared__ double lds[TileShape<TX, TY>::LDS_DOUBLES];
     const int b = blockIdx.x, nb = gridDim.x;  // nb is a multiple of 8
     const int tile = (b % 8) * (nb / 8) + b / 8;
     if (tile >= ntiles) return;
-    stage_tile<SOLVER, LIMITER, STAGE_TX, STAGE_TY>(g, stage, use_lim != 0, mood != 0,
-                                                   tile % tiles_x, tile / tiles_x, lds);
+    stage_tile<SOLVER, LIMITER, TX, TY>(g, stage, use_lim != 0, mood != 0, tile % tiles_x,
+                                        tile / tiles_x, lds);
 }
 
 using StageFn = void (*)(GridView, int, int, int, int, int);
 
-StageFn stage_fn(int solver, int limiter) {
-    static const StageFn table[3][2] = {
-        {stage_kernel<RUSANOV, LIM_VAN_LEER>, stage_kernel<RUSANOV, LIM_MC>},
-        {stage_kernel<HLLE, LIM_VAN_LEER>, stage_kernel<HLLE, LIM_MC>},
-        {stage_kernel<HLLD, LIM_VAN_LEER>, stage_kernel<HLLD, LIM_MC>}};
-    return table[solver - 1][limiter - 1];
+#define CH_SHAPES(S, L)                                                                 \
+    {stage_kernel<S, L, 32, 8>, stage_kernel<S, L, 16, 16>, stage_kernel<S, L, 64, 4>}
+
+StageFn stage_fn(int solver, int limiter, int shape) {
+    static const StageFn table[3][2][NUM_SHAPES] = {
+        {CH_SHAPES(RUSANOV, LIM_VAN_LEER), CH_SHAPES(RUSANOV, LIM_MC)},
+        {CH_SHAPES(HLLE, LIM_VAN_LEER), CH_SHAPES(HLLE, LIM_MC)},
+        {CH_SHAPES(HLLD, LIM_VAN_LEER), CH_SHAPES(HLLD, LIM_MC)}};
+    return table[solver - 1][limiter - 1][shape];
 }
+#undef CH_SHAPES
 
 __device__ __forceinline__ double wave_max(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(256) void divb_reduce_kernel(const double *bx, cons
 }
 
 int launch_stage(const StageLaunch &L, hipStream_t stream) {
-    StageFn fn = stage_fn(L.solver, L.limiter);
+    StageFn fn = stage_fn(L.solver, L.limiter, L.shape);
     hipLaunchKernelGGL(fn, dim3(L.grid), dim3(STAGE_THREADS), 0, stream, L.g, L.stage, L.use_lim,
                        L.mood, L.tiles_x, L.ntiles);
     return launch_check("stage_kernel");
@@ -245,6 +250,7 @@ struct chimera_ctx {
     int dt_cap = 0;
     unsigned long long *scratch_u64 = nullptr;  // [0] cfl / divb max, [1] divb sum (as double)
     uint32_t *mask = nullptr;
+    int shape = 0;
     int tiles_x = 0, tiles_y = 0, ntiles = 0, grid = 0;
     hipGraphExec_t gexec[2][3] = {};
     bool timing = false;
@@ -283,6 +289,25 @@ static int collect_events(chimera_ctx *c) {
 
 extern "C" const char *chimera_last_error(void) { return chimera::g_err; }
 
+static void drop_graphs(chimera_ctx *c) {
+    for (auto &row : c->gexec)
+        for (auto &g : row)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+}
+
+static void set_shape(chimera_ctx *c, int shape) {
+    c->shape = shape;
+    const TileDims d = TILE_SHAPES[shape];
+    c->tiles_x = (c->nx + d.tx - 1) / d.tx;
+    c->tiles_y = (c->ny + d.ty - 1) / d.ty;
+    c->ntiles = c->tiles_x * c->tiles_y;
+    c->grid = (c->ntiles + 7) / 8 * 8;
+    drop_graphs(c);
+}
+
 extern "C" int chimera_device_count(int *count) {
     CH_REQUIRE(count, "count is NULL");
     CH_HIP(hipGetDeviceCount(count));
@@ -311,6 +336,7 @@ static StageLaunch stage_launch(chimera_ctx *c, int stage, double *const *uin,
     L.mood = c->sch.fallback_2_mood;
     L.solver = c->sch.riemann_solver;
     L.limiter = c->sch.slope_limiter;
+    L.shape = c->shape;
     L.tiles_x = c->tiles_x;
     L.ntiles = c->ntiles;
     L.grid = c->grid;
@@ -393,10 +419,7 @@ extern "C" int chimera_ctx_create(chimera_ctx_t **out, int nx, int ny, double dx
     }
     for (int s = 0; s < 3; ++s)
         for (int q = 0; q < 6; ++q) c->S[s][q] = c->base + (size_t)(6 * s + q) * n;
-    c->tiles_x = (nx + STAGE_TX - 1) / STAGE_TX;
-    c->tiles_y = (ny + STAGE_TY - 1) / STAGE_TY;
-    c->ntiles = c->tiles_x * c->tiles_y;
-    c->grid = (c->ntiles + 7) / 8 * 8;
+    set_shape(c, 0);
     int rc = launch_set_state(c->st, 0.0, 1, 0, c->stream);
     if (rc == CHIMERA_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = CHIMERA_EHIP;
     if (rc != CHIMERA_OK) {
@@ -410,9 +433,7 @@ extern "C" int chimera_ctx_create(chimera_ctx_t **out, int nx, int ny, double dx
 extern "C" int chimera_ctx_destroy(chimera_ctx_t *c) {
     if (!c) return CHIMERA_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto &row : c->gexec)
-        for (auto &g : row)
-            if (g) (void)hipGraphExecDestroy(g);
+    drop_graphs(c);
     for (auto e : c->ev) (void)hipEventDestroy(e);
     (void)hipFree(c->base);
     (void)hipFree(c->st);
@@ -606,5 +627,56 @@ extern "C" int chimera_ctx_stage_kernel_ms(chimera_ctx_t *c, double *ms, int *la
     CH_TRY(collect_events(c));
     if (ms) *ms = c->stage_launches ? c->stage_ms / c->stage_launches : 0.0;
     if (launches) *launches = c->stage_launches;
+    return CHIMERA_OK;
+}
+
+extern "C" int chimera_ctx_set_tile_shape(chimera_ctx_t *c, int shape) {
+    CH_REQUIRE(c, "ctx is NULL");
+    CH_REQUIRE(shape >= 0 && shape < NUM_SHAPES, "tile shape must be 0..%d", NUM_SHAPES - 1);
+    CH_HIP(hipStreamSynchronize(c->stream));
+    set_shape(c, shape);
+    return CHIMERA_OK;
+}
+
+extern "C" int chimera_ctx_get_tile_shape(chimera_ctx_t *c, int *shape, int *tx, int *ty) {
+    CH_REQUIRE(c, "ctx is NULL");
+    if (shape) *shape = c->shape;
+    if (tx) *tx = TILE_SHAPES[c->shape].tx;
+    if (ty) *ty = TILE_SHAPES[c->shape].ty;
+    return CHIMERA_OK;
+}
+
+// Time each tile shape on one stage of the current state (output to the scratch
+// copy, the state is not advanced) and keep the fastest.  ms_out (optional) gets
+// NUM_SHAPES timings.
+extern "C" int chimera_ctx_autotune(chimera_ctx_t *c, int reps, int *best, double *ms_out) {
+    CH_REQUIRE(c, "ctx is NULL");
+    if (reps < 1) reps = 3;
+    hipEvent_t a, b;
+    CH_HIP(hipEventCreate(&a));
+    CH_HIP(hipEventCreate(&b));
+    double best_ms = 1e300;
+    int best_shape = c->shape;
+    for (int sh = 0; sh < NUM_SHAPES; ++sh) {
+        set_shape(c, sh);
+        StageLaunch L = stage_launch(c, 2, c->S[c->cur], c->S[c->cur], c->S[2], nullptr);
+        CH_TRY(launch_stage(L, c->stream));  // warm-up
+        CH_HIP(hipEventRecord(a, c->stream));
+        for (int r = 0; r < reps; ++r) CH_TRY(launch_stage(L, c->stream));
+        CH_HIP(hipEventRecord(b, c->stream));
+        CH_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        CH_HIP(hipEventElapsedTime(&ms, a, b));
+        double per = ms / reps;
+        if (ms_out) ms_out[sh] = per;
+        if (per < best_ms) {
+            best_ms = per;
+            best_shape = sh;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    set_shape(c, best_shape);
+    if (best) *best = best_shape;
     return CHIMERA_OK;
 }

@@ -10,11 +10,15 @@
 
 namespace chimera {
 
-// Stage tile: 32 x 8 cells, one cell per thread in the update phase; LDS holds the
-// primitives of the 36 x 12 halo tile and the 33 x 10 + 34 x 9 face fluxes (46 KiB).
-constexpr int STAGE_TX = 32;
-constexpr int STAGE_TY = 8;
-constexpr int STAGE_THREADS = STAGE_TX * STAGE_TY;
+// Tile shapes the stage kernels are instantiated for; every shape has 256 cells, one
+// per thread in the update phase.  32 x 8 keeps 46 KiB of LDS (primitives of the
+// 36 x 12 halo tile plus 33 x 10 x-face and 34 x 9 y-face fluxes).
+struct TileDims {
+    int tx, ty;
+};
+constexpr int NUM_SHAPES = 3;
+constexpr TileDims TILE_SHAPES[NUM_SHAPES] = {{32, 8}, {16, 16}, {64, 4}};
+constexpr int STAGE_THREADS = 256;
 
 // Per-grid step bookkeeping kept on the device so whole steps can be captured.
 struct StepState {
@@ -27,7 +31,7 @@ struct StepState {
 
 struct StageLaunch {
     GridView g;
-    int stage, use_lim, mood, solver, limiter;
+    int stage, use_lim, mood, solver, limiter, shape;
     int tiles_x, ntiles, grid;
 };
 

@@ -120,6 +120,21 @@ class Grid:
               "stage_kernel_ms")
         return ms.value, n.value
 
+    def set_tile_shape(self, shape: int) -> None:
+        check(self._L.chimera_ctx_set_tile_shape(self._h, int(shape)), "set_tile_shape")
+
+    def tile_shape(self) -> tuple[int, int, int]:
+        s, tx, ty = C.c_int(), C.c_int(), C.c_int()
+        check(self._L.chimera_ctx_get_tile_shape(self._h, C.byref(s), C.byref(tx), C.byref(ty)),
+              "get_tile_shape")
+        return s.value, tx.value, ty.value
+
+    def autotune(self, reps: int = 3) -> tuple[int, list[float]]:
+        best = C.c_int()
+        ms = (C.c_double * 3)()
+        check(self._L.chimera_ctx_autotune(self._h, int(reps), C.byref(best), ms), "autotune")
+        return best.value, list(ms)
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._L.chimera_ctx_destroy(self._h)

@@ -199,6 +199,46 @@ void *chimera_ctx_stream(chimera_ctx_t *ctx);
  * call, measured with HIP events on the context's stream (0 if not measured). */
 int chimera_ctx_set_timing(chimera_ctx_t *ctx, int enable);
 int chimera_ctx_stage_kernel_ms(chimera_ctx_t *ctx, double *ms, int *launches);
+/* Tile shape of the fused stage kernel: 0 = 32x8, 1 = 16x16, 2 = 64x4 cells. */
+int chimera_ctx_set_tile_shape(chimera_ctx_t *ctx, int shape);
+int chimera_ctx_get_tile_shape(chimera_ctx_t *ctx, int *shape, int *tx, int *ty);
+/* Time every tile shape on one stage of the current state (`reps` launches each),
+ * keep the fastest; ms_out (optional) receives one average per shape. */
+int chimera_ctx_autotune(chimera_ctx_t *ctx, int reps, int *best_shape, double *ms_out);
+
+/* ------------------------------------------------------------------------------
+ * 3a. Batched ensemble: many independent grids of uneven sizes advanced together.
+ *     Each RK stage is ONE persistent launch over a tile table spanning all members;
+ *     dt is per member (CFL-reduced on the device); steps can be replayed as hipGraphs.
+ *     Replaces the sequential loop of single runs in scripts/run_mhd_MC.py:57-73.
+ * ---------------------------------------------------------------------------- */
+typedef struct chimera_batch chimera_batch_t;
+
+/* nx, ny, dx, gamma: one entry per member; bc: one entry per member; scheme shared. */
+int chimera_batch_create(chimera_batch_t **batch, int members, const int *nx, const int *ny,
+                         const double *dx, const double *gamma, const chimera_bc_t *bc,
+                         const chimera_scheme_t *scheme, void *stream);
+int chimera_batch_destroy(chimera_batch_t *batch);
+/* state = {Mass, Momx, Momy, Energy, b_x, b_y} of one member; host or device pointers. */
+int chimera_batch_upload(chimera_batch_t *batch, int member, const double *const *state);
+int chimera_batch_download(chimera_batch_t *batch, int member, double *const *state);
+/* nsteps steps of every member (dt_mode CHIMERA_DT_REFERENCE or CHIMERA_DT_CFL). */
+int chimera_batch_steps(chimera_batch_t *batch, int nsteps, int dt_mode, int use_graph);
+int chimera_batch_member_time(chimera_batch_t *batch, int member, double *t, int *out_count,
+                              double *last_dt);
+int chimera_batch_sync(chimera_batch_t *batch);
+void *chimera_batch_stream(chimera_batch_t *batch);
+/* Total cells over all members. */
+int chimera_batch_cells(chimera_batch_t *batch, long long *cells);
+/* Tile shape (as chimera_ctx_set_tile_shape) and persistent blocks per CU (0 = auto). */
+int chimera_batch_set_tiling(chimera_batch_t *batch, int shape, int blocks_per_cu);
+int chimera_batch_get_tiling(chimera_batch_t *batch, int *shape, int *blocks_per_cu);
+/* Time every (tile shape, blocks per CU) candidate on one stage, keep the fastest. */
+int chimera_batch_autotune(chimera_batch_t *batch, int reps, int *best_shape,
+                           int *best_blocks_per_cu);
+/* Average fused-stage launch duration (ms) when timing is enabled (HIP events). */
+int chimera_batch_set_timing(chimera_batch_t *batch, int enable);
+int chimera_batch_stage_kernel_ms(chimera_batch_t *batch, double *ms, int *launches);
 
 #ifdef __cplusplus
 }

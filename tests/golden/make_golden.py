@@ -1,0 +1,66 @@
+#!/usr/bin/env python3
+"""Write tests/golden/ref_vectors.npz from the reference build (oracle/_ref).
+
+The vectors are outputs of the reference Fortran modules themselves (compiled by
+oracle/build_ref.sh with amdflang, no FMA contraction), on small seeded inputs:
+  - one and ten SSP-RK3 steps of the Orszag-Tang vortex (16^2, reference dt policy)
+    and five steps of the rotor (16^2, outflow) for every solver x limiter;
+  - compute_stage_fluxes on a seeded primitive field (periodic and outflow);
+  - compute_fluxes on seeded face states for each Riemann solver.
+They let the oracle be checked against the reference where the reference build is
+not available (e.g. on the GPU box).  Regenerate with
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from chimera_amd import problems  # noqa: E402
+from oracle.oracle import RefVariant, ref_available  # noqa: E402
+from tests.inputs import random_face_states, random_prims  # noqa: E402
+
+OUT = Path(__file__).resolve().parent / "ref_vectors.npz"
+N = 16
+
+
+def main() -> None:
+    if not ref_available():
+        raise SystemExit("oracle/_ref is not built (needs the reference sources)")
+    v: dict[str, np.ndarray] = {}
+    ot = problems.orszag_tang(N)
+    rot = problems.rotor(N)
+    v["ot_init"] = ot.state
+    v["rotor_init"] = rot.state
+    prim = random_prims(N, N, 2024)
+    v["prim"] = prim
+    fin = random_face_states(256, 77)
+    v["flux_in"] = fin
+    for s in (1, 2, 3):
+        ref = RefVariant(s, 2)
+        v[f"flux_s{s}"] = ref.compute_fluxes(fin, 1.4)
+        for l in (1, 2):
+            ref = RefVariant(s, l)
+            st = ot.state.copy()
+            r1 = ref.run(st, ot.dx, ot.gamma, ot.bc, 1)
+            v[f"ot_s{s}l{l}_1"] = st.copy()
+            r9 = ref.run(st, ot.dx, ot.gamma, ot.bc, 9, t=r1["t"], out_count=r1["out_count"])
+            v[f"ot_s{s}l{l}_10"] = st.copy()
+            v[f"ot_s{s}l{l}_dt"] = np.concatenate([r1["dt"], r9["dt"]])
+            st = rot.state.copy()
+            ref.run(st, rot.dx, rot.gamma, rot.bc, 5, dt_mode=1)
+            v[f"rotor_s{s}l{l}_5"] = st
+            for bcname, bc in (("periodic", (1, 1, 1, 1)), ("outflow", (3, 3, 3, 3))):
+                flux, _ = ref.stage_fluxes(prim, 1.0 / N, 5.0 / 3.0, bc)
+                v[f"stage_s{s}l{l}_{bcname}"] = flux
+    np.savez_compressed(OUT, **v)
+    print(f"wrote {OUT} ({OUT.stat().st_size / 1024:.0f} KiB, {len(v)} arrays)")
+
+
+if __name__ == "__main__":
+    main()
