@@ -43,6 +43,9 @@ def parse_args():
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="steps of the CPU baseline sample (0 disables it)")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay pass")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device copy of 2^26 doubles (512 MiB each way) 3 "
+                         "times, the reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -100,6 +103,13 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        a = torch.ones(1 << 26, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     n = args.n
     cells = n * n
     prob = problems.rotor(n, scheme=(2, 1, 1, 1))
