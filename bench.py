@@ -43,6 +43,11 @@ def parse_args():
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="steps of the CPU baseline sample (0 disables it)")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay pass")
+    ap.add_argument("--members", type=int, default=512,
+                    help="batched GRF ensemble size (0 skips the batched section)")
+    ap.add_argument("--grf-steps", type=int, default=500, help="steps of every GRF member")
+    ap.add_argument("--plain-per-class", type=int, default=8,
+                    help="members per grid size timed on the plain single-run path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of 2^26 doubles (512 MiB each way) 3 "
                          "times, the reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -76,6 +81,68 @@ def cpu_baseline(n: int, steps: int) -> dict | None:
     return {"value": n * n * steps / el, "unit": "cell-updates/s", "cores": threads,
             "kind": kind, "sample": f"rotor {n}x{n}, HLLE/Van Leer, {steps} SSP-RK3 steps "
             f"({el:.1f} s), {what}"}
+
+
+def batched_grf(args, torch) -> dict:
+    """Batched GRF ensemble (members of uneven size, Rusanov / MC, per-member CFL dt,
+    periodic): every member advanced together, one persistent launch per RK stage,
+    steps replayed as hipGraphs -- against the plain path, which advances the same
+    members one after another through the resident single-grid driver (also graph
+    replayed); the plain time is extrapolated from `plain_per_class` members per size."""
+    from chimera_amd import problems
+    from chimera_amd.batch import Ensemble, DT_CFL, SHAPES
+    from chimera_amd.solver import Grid
+    sizes = problems.grf_sizes(args.members)
+    probs = [problems.grf_member_device(n, 1000 + m, scheme=(1, 2, 1, 1))
+             for m, n in enumerate(sizes)]
+    torch.cuda.synchronize()
+    cells = sum(n * n for n in sizes)
+    steps = args.grf_steps
+    ens = Ensemble(probs)
+    shape, bpc = ens.autotune(reps=2)
+    ens.steps(2, dt_mode=DT_CFL, graph=True)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.steps(steps, dt_mode=DT_CFL, graph=True)
+    ens.sync()
+    el = time.perf_counter() - t0
+    ens.set_timing(True)
+    ens.steps(3, dt_mode=DT_CFL)
+    stage_ms, launches = ens.stage_kernel_ms()
+    ens.set_timing(False)
+    ens.close()
+    batched = cells * steps / el
+
+    per_class = {}
+    for n in sorted(set(sizes)):
+        idx = [m for m, s_ in enumerate(sizes) if s_ == n][:args.plain_per_class]
+        times = []
+        for m in idx:
+            with Grid.from_problem(probs[m]) as g:
+                g.steps(2, dt_mode=DT_CFL, graph=True)
+                g.sync()
+                t1 = time.perf_counter()
+                g.steps(steps, dt_mode=DT_CFL, graph=True)
+                g.sync()
+                times.append(time.perf_counter() - t1)
+        per_class[n] = (sizes.count(n), len(idx), sum(times) / len(times))
+    plain_time = sum(cnt * t for cnt, _, t in per_class.values())
+    plain = cells * steps / plain_time
+    achieved = BYTES_PER_CELL_STAGE * cells / (stage_ms * 1e-3) / 1e9 if stage_ms else None
+    del probs
+    torch.cuda.empty_cache()
+    return {"workload": f"GRF ensemble: {args.members} members, N in {{128,256,384,512}} "
+                        f"(seed 20261203), Rusanov, MC, MUSCL+MOOD, CT, per-member CFL dt, "
+                        f"{steps} steps each, periodic",
+            "cells": cells, "steps": steps, "value": batched, "plain_value": plain,
+            "ratio": batched / plain, "unit": "cell-updates/s",
+            "batched_ms_per_step": el * 1e3 / steps,
+            "plain_extrapolated_s": plain_time,
+            "plain_members_timed": {str(n): v[1] for n, v in per_class.items()},
+            "tiling": {"tile": SHAPES[shape], "blocks_per_cu": bpc},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                         "avg_launch_ms": stage_ms, "launches": launches}}
 
 
 def traffic_from_profiles(n: int) -> float | None:
@@ -141,6 +208,8 @@ def main():
     finite = bool(torch.isfinite(torch.from_numpy(st)).all())
     g.close()
 
+    batched = batched_grf(args, torch) if args.members > 0 else None
+
     value = cells * args.steps / elapsed
     achieved = BYTES_PER_CELL_STAGE * cells / (stage_ms * 1e-3) / 1e9 if stage_ms > 0 else None
     out = {
@@ -168,6 +237,7 @@ def main():
         "cpu_baseline": cpu,
         "graph_value": graph_value,
         "finite": finite,
+        "batched_grf": batched,
     }
     print(json.dumps(out))
 

@@ -44,10 +44,16 @@ class Ensemble:
             for k, p in enumerate(probs):
                 self.upload(k, p.state)
 
-    def upload(self, member: int, state: np.ndarray) -> None:
-        st = np.ascontiguousarray(state, dtype=np.float64)
-        assert st.shape == (6,) + self.shapes[member]
-        ptrs = (C.c_void_p * 6)(*[st[q].ctypes.data for q in range(6)])
+    def upload(self, member: int, state) -> None:
+        """state: numpy (6, ny, nx) float64 or a CUDA float64 tensor of that shape."""
+        if isinstance(state, np.ndarray):
+            st = np.ascontiguousarray(state, dtype=np.float64)
+            assert st.shape == (6,) + self.shapes[member]
+            ptrs = (C.c_void_p * 6)(*[st[q].ctypes.data for q in range(6)])
+        else:
+            st = state.contiguous()
+            assert tuple(st.shape) == (6,) + self.shapes[member] and str(st.dtype) == "torch.float64"
+            ptrs = (C.c_void_p * 6)(*[st[q].data_ptr() for q in range(6)])
         check(self._L.chimera_batch_upload(self._h, int(member), ptrs), "chimera_batch_upload")
 
     def download(self, member: int) -> np.ndarray:

@@ -50,10 +50,17 @@ struct TileShape {
     static constexpr int LDS_DOUBLES = 6 * P_SZ + 5 * FX_SZ + 5 * FY_SZ;
 };
 
+// Global index of a halo position: periodic wrap or zero-gradient clamp.  Halo
+// positions lie in [-2, n + T + 1], so one conditional add / subtract wraps them
+// whenever n >= T + 2; smaller grids fall back to the modulo.
 __device__ __forceinline__ int map_index(int g, int n, int periodic) {
     if (periodic) {
-        int r = g % n;
-        return r < 0 ? r + n : r;
+        if (g < 0) g += n;
+        if (g >= n) {
+            g -= n;
+            if (g >= n) g %= n;
+        }
+        return g;
     }
     return g < 0 ? 0 : (g >= n ? n - 1 : g);
 }
@@ -150,73 +157,75 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
 
     const bool want_mask = g.mask != nullptr;
 
-    // ---- phase 2a: x faces ------------------------------------------------------
-    for (int idx = tid; idx < S::FX_SZ; idx += NT) {
-        int fx = idx % S::FXW, fy = idx / S::FXW;      // face i = fx-1, row j = fy-1
-        int lx = fx + 1, ly = fy + 1;                  // LDS position of the left cell
-        int ci = map_index(x0 + fx - 1, nx, px);
-        bool last = !px && ci == nx - 1;
-        double L[6], R[6], C[6];
-        unsigned bits = 0;
+    // ---- phase 2: x faces, then y faces (rotated: normal vy / By), in one loop so
+    //      that no wave idles between the two face sets ----------------------------
+    for (int face = tid; face < S::FX_SZ + S::FY_SZ; face += NT) {
+        if (face < S::FX_SZ) {
+            const int idx = face;
+            int fx = idx % S::FXW, fy = idx / S::FXW;  // face i = fx-1, row j = fy-1
+            int lx = fx + 1, ly = fy + 1;                  // LDS position of the left cell
+            int ci = map_index(x0 + fx - 1, nx, px);
+            bool last = !px && ci == nx - 1;
+            double L[6], R[6], C[6];
+            unsigned bits = 0;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const double *row = sP + q * S::P_SZ + ly * S::PW;
-            double c = row[lx];
-            C[q] = c;
-            bits |= face_pair<LIMITER>(row[lx - 1], c, row[lx + 1], row[lx + 2], row[lx + S::PW],
-                                       row[lx - S::PW], last, use_lim, mood,
-                                       q == F_RHO || q == F_P, inv_dx, inv_2dx, hdx, L[q], R[q])
-                    << (4 * q);
-        }
-        if (thermal_reset(L, C)) bits |= 1u << 24;
-        if (thermal_reset(R, C)) bits |= 1u << 25;
-        FaceState a{L[F_RHO], L[F_VX], L[F_VY], L[F_P], L[F_BX], L[F_BY]};
-        FaceState b{R[F_RHO], R[F_VX], R[F_VY], R[F_P], R[F_BX], R[F_BY]};
-        Flux5 f = riemann<SOLVER>(a, b, gam);
-        sFX[0 * S::FX_SZ + idx] = f.mass;
-        sFX[1 * S::FX_SZ + idx] = f.mom_n;
-        sFX[2 * S::FX_SZ + idx] = f.mom_t;
-        sFX[3 * S::FX_SZ + idx] = f.energy;
-        sFX[4 * S::FX_SZ + idx] = f.bt;
-        if (want_mask) {
-            int i = fx - 1, j = fy - 1;
-            if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny && bits)
-                atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
-        }
-    }
-
-    // ---- phase 2b: y faces (rotated: normal vy / By) ----------------------------
-    for (int idx = tid; idx < S::FY_SZ; idx += NT) {
-        int fx = idx % S::FYW, fy = idx / S::FYW;      // face column i = fx-1, j = fy-1
-        int lx = fx + 1, ly = fy + 1;                  // LDS position of the lower cell
-        int cj = map_index(y0 + fy - 1, ny, py);
-        bool last = !py && cj == ny - 1;
-        double L[6], R[6], C[6];
-        unsigned bits = 0;
+            for (int q = 0; q < 6; ++q) {
+                const double *row = sP + q * S::P_SZ + ly * S::PW;
+                double c = row[lx];
+                C[q] = c;
+                bits |= face_pair<LIMITER>(row[lx - 1], c, row[lx + 1], row[lx + 2], row[lx + S::PW],
+                                           row[lx - S::PW], last, use_lim, mood,
+                                           q == F_RHO || q == F_P, inv_dx, inv_2dx, hdx, L[q], R[q])
+                        << (4 * q);
+            }
+            if (thermal_reset(L, C)) bits |= 1u << 24;
+            if (thermal_reset(R, C)) bits |= 1u << 25;
+            FaceState a{L[F_RHO], L[F_VX], L[F_VY], L[F_P], L[F_BX], L[F_BY]};
+            FaceState b{R[F_RHO], R[F_VX], R[F_VY], R[F_P], R[F_BX], R[F_BY]};
+            Flux5 f = riemann<SOLVER>(a, b, gam);
+            sFX[0 * S::FX_SZ + idx] = f.mass;
+            sFX[1 * S::FX_SZ + idx] = f.mom_n;
+            sFX[2 * S::FX_SZ + idx] = f.mom_t;
+            sFX[3 * S::FX_SZ + idx] = f.energy;
+            sFX[4 * S::FX_SZ + idx] = f.bt;
+            if (want_mask) {
+                int i = fx - 1, j = fy - 1;
+                if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny && bits)
+                    atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
+            }
+        } else {
+            const int idx = face - S::FX_SZ;
+            int fx = idx % S::FYW, fy = idx / S::FYW;      // face column i = fx-1, j = fy-1
+            int lx = fx + 1, ly = fy + 1;                  // LDS position of the lower cell
+            int cj = map_index(y0 + fy - 1, ny, py);
+            bool last = !py && cj == ny - 1;
+            double L[6], R[6], C[6];
+            unsigned bits = 0;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const double *col = sP + q * S::P_SZ + ly * S::PW + lx;
-            double c = col[0];
-            C[q] = c;
-            bits |= face_pair<LIMITER>(col[-S::PW], c, col[S::PW], col[2 * S::PW], col[1], col[-1],
-                                       last, use_lim, mood, q == F_RHO || q == F_P, inv_dx,
-                                       inv_2dx, hdx, L[q], R[q])
-                    << (4 * q + 2);
-        }
-        if (thermal_reset(L, C)) bits |= 1u << 26;
-        if (thermal_reset(R, C)) bits |= 1u << 27;
-        FaceState a{L[F_RHO], L[F_VY], L[F_VX], L[F_P], L[F_BY], L[F_BX]};
-        FaceState b{R[F_RHO], R[F_VY], R[F_VX], R[F_P], R[F_BY], R[F_BX]};
-        Flux5 f = riemann<SOLVER>(a, b, gam);
-        sFY[0 * S::FY_SZ + idx] = f.mass;
-        sFY[1 * S::FY_SZ + idx] = f.mom_t;   // Momx through y faces
-        sFY[2 * S::FY_SZ + idx] = f.mom_n;   // Momy through y faces
-        sFY[3 * S::FY_SZ + idx] = f.energy;
-        sFY[4 * S::FY_SZ + idx] = f.bt;      // Bx flux through y faces
-        if (want_mask) {
-            int i = fx - 1, j = fy - 1;
-            if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny && bits)
-                atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
+            for (int q = 0; q < 6; ++q) {
+                const double *col = sP + q * S::P_SZ + ly * S::PW + lx;
+                double c = col[0];
+                C[q] = c;
+                bits |= face_pair<LIMITER>(col[-S::PW], c, col[S::PW], col[2 * S::PW], col[1], col[-1],
+                                           last, use_lim, mood, q == F_RHO || q == F_P, inv_dx,
+                                           inv_2dx, hdx, L[q], R[q])
+                        << (4 * q + 2);
+            }
+            if (thermal_reset(L, C)) bits |= 1u << 26;
+            if (thermal_reset(R, C)) bits |= 1u << 27;
+            FaceState a{L[F_RHO], L[F_VY], L[F_VX], L[F_P], L[F_BY], L[F_BX]};
+            FaceState b{R[F_RHO], R[F_VY], R[F_VX], R[F_P], R[F_BY], R[F_BX]};
+            Flux5 f = riemann<SOLVER>(a, b, gam);
+            sFY[0 * S::FY_SZ + idx] = f.mass;
+            sFY[1 * S::FY_SZ + idx] = f.mom_t;   // Momx through y faces
+            sFY[2 * S::FY_SZ + idx] = f.mom_n;   // Momy through y faces
+            sFY[3 * S::FY_SZ + idx] = f.energy;
+            sFY[4 * S::FY_SZ + idx] = f.bt;      // Bx flux through y faces
+            if (want_mask) {
+                int i = fx - 1, j = fy - 1;
+                if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny && bits)
+                    atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
+            }
         }
     }
     __syncthreads();

@@ -249,3 +249,51 @@ def grf_sizes(members: int, choices=(128, 256, 384, 512), seed: int = 20261203) 
     """Grid size of each ensemble member, drawn uniformly from ``choices``."""
     rng = np.random.default_rng(seed)
     return [int(c) for c in rng.choice(choices, size=members)]
+
+
+def grf_member_device(n: int, seed: int, scheme=(1, 2, 1, 1), device="cuda", **kw):
+    """The ensemble member of ``grf_member`` generated on the GPU (torch FFTs): the
+    physical parameters come from the same numpy draw, the two white-noise fields from
+    a torch generator seeded with ``seed``.  The returned Problem's ``state`` is a
+    float64 tensor (6, n, n) on ``device``."""
+    import torch
+    rng = np.random.default_rng(seed)
+    gamma = rng.uniform(*kw.get("gamma_range", (1.4, 5.0 / 3.0)))
+    v0 = rng.uniform(*kw.get("v0_range", (0.1, 1.0)))
+    beta = rng.uniform(*kw.get("beta_range", (0.1, 10.0)))
+    ell = rng.uniform(*kw.get("ell_range", (0.05, 0.15)))
+    cf_cap = kw.get("cf_cap", 3.0)
+    dx = 1.0 / n
+    gen = torch.Generator(device=device).manual_seed(int(seed))
+    k = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float64) * TWO_PI
+    S = torch.exp(-0.5 * ell * ell * (k[None, :] ** 2 + k[:, None] ** 2))
+
+    def field():
+        w = torch.randn((n, n), generator=gen, device=device, dtype=torch.float64)
+        f = torch.fft.ifft2(torch.fft.fft2(w) * S).real
+        f = f - f.mean()
+        return f / torch.sqrt(torch.mean(f * f))
+
+    psi, Az = field(), field()
+    vx = (torch.roll(psi, -1, 0) - torch.roll(psi, 1, 0)) / (2.0 * dx)
+    vy = -(torch.roll(psi, -1, 1) - torch.roll(psi, 1, 1)) / (2.0 * dx)
+    v_rms = torch.sqrt(torch.mean(vx * vx + vy * vy))
+    vx, vy = vx * (v0 / v_rms), vy * (v0 / v_rms)
+    inv_dx = 1.0 / dx
+    bx = (Az - torch.roll(Az, 1, 0)) * inv_dx
+    by = -((Az - torch.roll(Az, 1, 1)) * inv_dx)
+    Bx = 0.5 * (bx + torch.roll(bx, 1, 1))
+    By = 0.5 * (by + torch.roll(by, 1, 0))
+    s = np.sqrt(2.0 / beta) / torch.sqrt(torch.mean(Bx * Bx + By * By))
+    bx, by, Bx, By = bx * s, by * s, Bx * s, By * s
+    cf = torch.sqrt(gamma + Bx * Bx + By * By)
+    tmax = torch.max(cf + torch.sqrt(vx * vx + vy * vy))
+    sc = torch.clamp(cf_cap / tmax, max=1.0)
+    vx, vy = vx * sc, vy * sc
+    vol = dx * dx
+    half_b2 = 0.5 * (Bx * Bx + By * By)
+    M = torch.full_like(vx, vol)
+    E = (1.0 / (gamma - 1.0) + 0.5 * (vx * vx + vy * vy) + half_b2) * vol
+    st = torch.stack([M, M * vx, M * vy, E, bx, by]).contiguous()
+    return Problem(f"grf_{n}_{seed}", st, dx, gamma, PERIODIC, tuple(scheme),
+                   {"seed": seed, "ell": ell, "v0": v0, "beta": beta})
