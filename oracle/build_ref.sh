@@ -13,11 +13,19 @@
 #
 # Flags: -O3 -fopenmp -ffp-contract=off (no FMA contraction, so the bits are
 # comparable with the C oracle and the HIP kernels).
+#
+# When chimera_amd/libchimera_hip.so exists, each variant is also linked a second
+# time with the product's Fortran drop-in modules (chimera_amd/fortran/mhd_*.f90)
+# in place of the five reference modules:
+#   oracle/_ref/libchimera_dropin_s<solver>_l<limiter>.so
+# -- the same harness and the reference's mhd_config, every routine on the GPU.
 set -euo pipefail
 
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 SRC="${CHIMERA_REF_SRC:-/root/reference/src}"
 OUT="$HERE/_ref"
+SHIMS="$HERE/../chimera_amd/fortran"
+LIBHIP="$HERE/../chimera_amd/libchimera_hip.so"
 FC="${FC:-amdflang}"
 FFLAGS="-O3 -fPIC -fopenmp -ffp-contract=off"
 
@@ -53,6 +61,22 @@ for s in 1 2 3; do
       objs+=("$o")
     done
     $FC -shared -fopenmp -o "$OUT/libchimera_ref_s${s}_l${l}.so" "${objs[@]}"
+
+    if [ -f "$LIBHIP" ]; then
+      D="$B/dropin"
+      mkdir -p "$D"
+      dobjs=()
+      for m in "$B/mhd_config.f90" "$SHIMS/chimera_c.f90" "$SHIMS/mhd_field_ops.f90" \
+               "$SHIMS/mhd_change_states.f90" "$SHIMS/mhd_derivatives.f90" "$SHIMS/mhd_bc.f90" \
+               "$SHIMS/mhd_flux.f90" "$HERE/ref_harness.f90"; do
+        o="$D/$(basename "$m" .f90).o"
+        $FC $FFLAGS -module-dir "$D" -I "$D" -c "$m" -o "$o"
+        dobjs+=("$o")
+      done
+      $FC -shared -fopenmp -o "$OUT/libchimera_dropin_s${s}_l${l}.so" "${dobjs[@]}" \
+          -L"$(dirname "$LIBHIP")" -lchimera_hip -Wl,-rpath,'$ORIGIN/../../chimera_amd'
+    fi
   done
 done
-echo "build_ref.sh: built $(ls "$OUT"/libchimera_ref_*.so | wc -l) variants into $OUT"
+echo "build_ref.sh: built $(ls "$OUT"/libchimera_ref_*.so | wc -l) reference and" \
+     "$(ls "$OUT"/libchimera_dropin_*.so 2>/dev/null | wc -l) drop-in variants into $OUT"

@@ -232,16 +232,25 @@ def ref_available() -> bool:
                for s in (1, 2, 3) for l in (1, 2))
 
 
+def dropin_available() -> bool:
+    return all((REF_DIR / f"libchimera_dropin_s{s}_l{l}.so").exists()
+               for s in (1, 2, 3) for l in (1, 2))
+
+
 class RefVariant:
-    """One amdflang build of the reference modules (fixed solver / limiter)."""
+    """One amdflang build of the reference modules (fixed solver / limiter).
+
+    kind="ref": the reference modules; kind="dropin": the same harness and the
+    reference's mhd_config linked with the product's Fortran drop-in modules
+    (chimera_amd/fortran), so every routine runs on the GPU."""
 
     _cache: dict = {}
 
-    def __new__(cls, solver: int, limiter: int):
-        key = (solver, limiter)
+    def __new__(cls, solver: int, limiter: int, kind: str = "ref"):
+        key = (solver, limiter, kind)
         if key not in cls._cache:
             obj = super().__new__(cls)
-            path = REF_DIR / f"libchimera_ref_s{solver}_l{limiter}.so"
+            path = REF_DIR / f"libchimera_{kind}_s{solver}_l{limiter}.so"
             if not path.exists():
                 raise FileNotFoundError(f"{path} missing: run `make -C oracle ref`")
             obj.lib = C.CDLL(str(path), mode=os.RTLD_LOCAL)
