@@ -126,11 +126,15 @@ def _declare(L: C.CDLL) -> None:
         "chimera_batch_autotune": [vp, i, C.POINTER(i), C.POINTER(i)],
         "chimera_batch_set_timing": [vp, i],
         "chimera_batch_stage_kernel_ms": [vp, C.POINTER(d), C.POINTER(i)],
+        "chimera_adj_create": [C.POINTER(vp), i, i, d, d, pBC, pSch, vp],
+        "chimera_adj_destroy": [vp],
+        "chimera_adj_forward_step": [vp, C.POINTER(vp), d, C.POINTER(vp)],
+        "chimera_adj_step_vjp": [vp, C.POINTER(vp), d, C.POINTER(vp), C.POINTER(vp)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = C.c_int
-    for name in ("chimera_ctx_stream", "chimera_batch_stream"):
+    for name in ("chimera_ctx_stream", "chimera_batch_stream", "chimera_adj_stream"):
         getattr(L, name).argtypes = [vp]
         getattr(L, name).restype = vp

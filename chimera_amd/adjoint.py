@@ -1,0 +1,179 @@
+"""Differentiable SSP-RK3 evolution: torch.autograd.Function over the hand-written
+adjoint kernels (chimera_adj_*), with checkpointed recomputation.
+
+The dt sequence is frozen (recorded once, e.g. by ``record_dt``, and used by the
+forward pass, the adjoint and any finite-difference check alike), so the gradient
+is that of the discrete map U_0 -> U_N with the branches the forward pass took.
+
+    dts = record_dt(prob, 1000)
+    U0 = field_loop_state(Az, dx)                  # differentiable in torch
+    UN = evolve(U0, dts, prob, checkpoint_every=50)
+    loss = ((cell_b2(UN) - target) ** 2).sum(); loss.backward()
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import BC, Scheme, check
+from .solver import DT_CFL, DT_PRESCRIBED, Grid
+
+
+def _ptrs(t: torch.Tensor):
+    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.shape[0] == 6
+    return (C.c_void_p * 6)(*[t[q].data_ptr() for q in range(6)])
+
+
+class AdjointStepper:
+    """chimera_adj_t: one-step forward and VJP on device tensors (6, ny, nx)."""
+
+    def __init__(self, nx, ny, dx, gamma, bc, scheme):
+        L = _lib.lib()
+        self._L = L
+        self.nx, self.ny = int(nx), int(ny)
+        h = C.c_void_p()
+        check(L.chimera_adj_create(C.byref(h), self.nx, self.ny, float(dx), float(gamma),
+                                   C.byref(BC.of(bc)), C.byref(Scheme.of(scheme)), None),
+              "chimera_adj_create")
+        self._h = h
+
+    def forward_step(self, x: torch.Tensor, dt: float, out: torch.Tensor | None = None):
+        out = torch.empty_like(x) if out is None else out
+        torch.cuda.synchronize()
+        check(self._L.chimera_adj_forward_step(self._h, _ptrs(x), float(dt), _ptrs(out)),
+              "chimera_adj_forward_step")
+        return out
+
+    def vjp(self, x: torch.Tensor, dt: float, lam_out: torch.Tensor, out=None):
+        out = torch.empty_like(x) if out is None else out
+        torch.cuda.synchronize()
+        check(self._L.chimera_adj_step_vjp(self._h, _ptrs(x), float(dt), _ptrs(lam_out.contiguous()),
+                                           _ptrs(out)), "chimera_adj_step_vjp")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.chimera_adj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def record_dt(prob, nsteps: int, dt_mode: int = DT_CFL) -> np.ndarray:
+    """The dt of every step of a CFL-driven run of ``prob`` (resident GPU path)."""
+    dts = np.empty(nsteps)
+    with Grid.from_problem(prob) as g:
+        for s in range(nsteps):
+            g.steps(1, dt_mode=dt_mode)
+            dts[s] = g.last_dt()
+    return dts
+
+
+class _Evolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, U0, dts, geom, checkpoint_every):
+        nx, ny, dx, gamma, bc, scheme = geom
+        U0 = U0.detach().contiguous()
+        n = len(dts)
+        K = max(1, int(checkpoint_every))
+        ckpts = [U0.clone()]
+        with Grid(nx, ny, dx, gamma, bc, scheme) as g:
+            g.upload(U0)
+            for s0 in range(0, n, K):
+                k = min(K, n - s0)
+                g.steps(k, dt_mode=DT_PRESCRIBED, dt=dts[s0:s0 + k])
+                if s0 + k < n:
+                    ckpts.append(g.download(torch.empty_like(U0)))
+            UN = g.download(torch.empty_like(U0))
+        ctx.save_for_backward(*ckpts)
+        ctx.dts, ctx.geom, ctx.K = dts, geom, K
+        return UN
+
+    @staticmethod
+    def backward(ctx, gN):
+        ckpts = ctx.saved_tensors
+        dts, K = ctx.dts, ctx.K
+        nx, ny, dx, gamma, bc, scheme = ctx.geom
+        n = len(dts)
+        adj = AdjointStepper(nx, ny, dx, gamma, bc, scheme)
+        lam = gN.contiguous().clone()
+        buf = torch.empty_like(lam)
+        try:
+            for seg in reversed(range(len(ckpts))):
+                s0 = seg * K
+                s1 = min(s0 + K, n)
+                states = [ckpts[seg]]
+                for s in range(s0, s1 - 1):
+                    states.append(adj.forward_step(states[-1], dts[s]))
+                for s in reversed(range(s0, s1)):
+                    adj.vjp(states[s - s0], dts[s], lam, out=buf)
+                    lam, buf = buf, lam
+        finally:
+            adj.close()
+        return lam, None, None, None
+
+
+def evolve(U0: torch.Tensor, dts, prob_or_geom, checkpoint_every: int = 50) -> torch.Tensor:
+    """U_N after len(dts) SSP-RK3 steps with the given dt sequence; differentiable
+    with respect to U0 (float64 CUDA tensor (6, ny, nx))."""
+    if hasattr(prob_or_geom, "dx"):
+        p = prob_or_geom
+        geom = (p.nx, p.ny, p.dx, p.gamma, tuple(p.bc), tuple(p.scheme))
+    else:
+        geom = tuple(prob_or_geom)
+    dts = np.ascontiguousarray(dts, dtype=np.float64)
+    return _Evolve.apply(U0, dts, geom, checkpoint_every)
+
+
+# ---- differentiable set-up and diagnostics (torch ops) -------------------------
+
+def curl_2d_periodic(Az: torch.Tensor, dx: float):
+    """compute_curl_2d (src/mhd_field_ops.f90:15-57) on a periodic grid, in torch."""
+    inv_dx = 1.0 / dx
+    bx = (Az - torch.roll(Az, 1, 0)) * inv_dx
+    by = -((Az - torch.roll(Az, 1, 1)) * inv_dx)
+    return bx, by
+
+
+def cell_b_periodic(bx: torch.Tensor, by: torch.Tensor):
+    """average_face_to_cell_B (src/mhd_field_ops.f90:120-156), periodic, in torch."""
+    return 0.5 * (bx + torch.roll(bx, 1, 1)), 0.5 * (by + torch.roll(by, 1, 0))
+
+
+def field_loop_state(Az: torch.Tensor, dx: float, gamma: float = 5.0 / 3.0,
+                     hydro=None) -> torch.Tensor:
+    """State (6, ny, nx) of the field-loop problem with face B = curl(Az),
+    differentiable in Az: the torch form of problems.field_loop_from_az, operation
+    for operation.  ``hydro``: optional (rho, vx, vy, p_thermal) arrays; default
+    rho = p = 1, v = (2, 1)."""
+    bx, by = curl_2d_periodic(Az, dx)
+    Bx, By = cell_b_periodic(bx, by)
+    vol = dx * dx
+    if hydro is None:
+        rho = torch.ones_like(Az)
+        vx = torch.full_like(Az, 2.0)
+        vy = torch.full_like(Az, 1.0)
+        pth = torch.ones_like(Az)
+    else:
+        rho, vx, vy, pth = (torch.as_tensor(a, dtype=Az.dtype, device=Az.device) for a in hydro)
+    half_b2 = 0.5 * (Bx * Bx + By * By)
+    P = pth + half_b2
+    r = torch.clamp_min(rho, 1.0e-12)
+    M = r * vol
+    # divide by a tensor: on the GPU torch turns division by a Python scalar into a
+    # multiplication by its reciprocal, which is not the correctly rounded quotient
+    gm1 = torch.full_like(P, gamma - 1.0)
+    E = ((P - half_b2) / gm1 + 0.5 * r * (vx * vx + vy * vy) + half_b2) * vol
+    return torch.stack([M, M * vx, M * vy, E, bx, by])
+
+
+def cell_b2(U: torch.Tensor) -> torch.Tensor:
+    Bx, By = cell_b_periodic(U[4], U[5])
+    return Bx * Bx + By * By

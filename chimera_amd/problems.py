@@ -172,17 +172,50 @@ def field_loop(nx: int = 512, ny: int = 256, Az_scale=None, scheme=(2, 2, 1, 1),
     return (prob, Az) if return_az else prob
 
 
-def field_loop_from_az(Az: np.ndarray, dx: float, scheme=(2, 2, 1, 1)) -> Problem:
+def field_loop_from_az(Az: np.ndarray, dx: float, scheme=(2, 2, 1, 1), hydro=None) -> Problem:
+    """Field-loop state for a given corner potential Az (periodic).  ``hydro``
+    (optional (rho, vx, vy, p_thermal) arrays) replaces the uniform rho = p = 1,
+    v = (2, 1) background."""
     ny, nx = Az.shape
     bx, by = curl_2d(Az, dx, PERIODIC)
     Bx, By = face_to_cell_B(bx, by, PERIODIC)
-    rho = np.ones((ny, nx))
-    vx = np.full((ny, nx), 2.0)
-    vy = np.full((ny, nx), 1.0)
-    P = 1.0 + 0.5 * (Bx * Bx + By * By)
+    if hydro is None:
+        rho = np.ones((ny, nx))
+        vx = np.full((ny, nx), 2.0)
+        vy = np.full((ny, nx), 1.0)
+        pth = np.ones((ny, nx))
+    else:
+        rho, vx, vy, pth = hydro
+    P = pth + 0.5 * (Bx * Bx + By * By)
     gamma = 5.0 / 3.0
     st = _pack(*prim_to_cons(rho, vx, vy, P, Bx, By, gamma, dx * dx), bx, by)
     return Problem("field_loop", st, dx, gamma, PERIODIC, tuple(scheme))
+
+
+def smooth_periodic(nx: int, ny: int, seed: int, modes: int = 3) -> np.ndarray:
+    """A smooth periodic field of unit-order amplitude: a few Fourier modes with random
+    amplitudes and phases, so that no two neighbouring cells are equal by symmetry."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    f = np.zeros((ny, nx))
+    for kx in range(0, modes + 1):
+        for ky in range(-modes, modes + 1):
+            if kx == 0 and ky <= 0:
+                continue
+            a = rng.standard_normal() / (kx * kx + ky * ky)
+            f += a * np.cos(TWO_PI * (kx * X + ky * Y) + rng.uniform(0.0, TWO_PI))
+    return f / np.abs(f).max()
+
+
+def field_loop_background(nx: int, ny: int, seed: int = 11, amp: float = 0.1):
+    """(rho, vx, vy, p_thermal) = (1, 2, 1, 1) each times (1 + amp * smooth field):
+    a background with no uniform field, so that the discrete map is differentiable
+    everywhere (no limiter ties between equal neighbours)."""
+    f = [smooth_periodic(nx, ny, seed + q) for q in range(4)]
+    return (1.0 + amp * f[0], 2.0 * (1.0 + amp * f[1]), 1.0 * (1.0 + amp * f[2]),
+            1.0 + amp * f[3])
 
 
 # ---- Gaussian-random-field ensemble -------------------------------------------

@@ -240,6 +240,28 @@ int chimera_batch_autotune(chimera_batch_t *batch, int reps, int *best_shape,
 int chimera_batch_set_timing(chimera_batch_t *batch, int enable);
 int chimera_batch_stage_kernel_ms(chimera_batch_t *batch, double *ms, int *launches);
 
+/* ------------------------------------------------------------------------------
+ * 3b. Adjoint (discrete, frozen dt): reverse of one SSP-RK3 step.  Hand-written
+ *     reverse kernels for reconstruction (limiter, MOOD, face floors, thermal check),
+ *     Riemann fluxes, update, CT, outflow correction, floors and blends, taking the
+ *     branches the forward pass took.  All pointers are device pointers (6 arrays of
+ *     nx*ny: Mass, Momx, Momy, Energy, b_x, b_y).  No counterpart in the reference,
+ *     which has no derivatives.
+ * ---------------------------------------------------------------------------- */
+typedef struct chimera_adj chimera_adj_t;
+
+int chimera_adj_create(chimera_adj_t **adj, int nx, int ny, double dx, double gamma,
+                       const chimera_bc_t *bc, const chimera_scheme_t *scheme, void *stream);
+int chimera_adj_destroy(chimera_adj_t *adj);
+/* state_out = one step of state_in with the given dt (the adjoint's forward kernels,
+ * bit-identical to the fused path). */
+int chimera_adj_forward_step(chimera_adj_t *adj, const double *const *state_in, double dt,
+                             double *const *state_out);
+/* lam_in = (dU^{n+1}/dU^n)^T lam_out at U^n = state_n with dt frozen. */
+int chimera_adj_step_vjp(chimera_adj_t *adj, const double *const *state_n, double dt,
+                         const double *const *lam_out, double *const *lam_in);
+void *chimera_adj_stream(chimera_adj_t *adj);
+
 #ifdef __cplusplus
 }
 #endif
