@@ -100,6 +100,7 @@ def _declare(L: C.CDLL) -> None:
         "chimera_ctx_upload": [vp, C.POINTER(vp)],
         "chimera_ctx_download": [vp, C.POINTER(vp)],
         "chimera_ctx_steps": [vp, i, i, c_dp, i],
+        "chimera_ctx_steps_record": [vp, i, c_dp, C.POINTER(vp)],
         "chimera_ctx_get_time": [vp, C.POINTER(d), C.POINTER(i)],
         "chimera_ctx_set_time": [vp, d, i],
         "chimera_ctx_last_dt": [vp, C.POINTER(d)],
@@ -130,6 +131,7 @@ def _declare(L: C.CDLL) -> None:
         "chimera_adj_destroy": [vp],
         "chimera_adj_forward_step": [vp, C.POINTER(vp), d, C.POINTER(vp)],
         "chimera_adj_step_vjp": [vp, C.POINTER(vp), d, C.POINTER(vp), C.POINTER(vp)],
+        "chimera_adj_segment_vjp": [vp, C.POINTER(vp), c_dp, i, C.POINTER(vp)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -54,6 +54,17 @@ class AdjointStepper:
                                            _ptrs(out)), "chimera_adj_step_vjp")
         return out
 
+    def segment_vjp(self, x0: torch.Tensor, dts, lam: torch.Tensor) -> torch.Tensor:
+        """lam <- (dU^k / dU^0)^T lam in place, for the k = len(dts) steps from x0 = U^0:
+        the states are recomputed with the fused forward and the VJPs run back to back
+        on the device (one call per checkpoint segment)."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        assert lam.is_contiguous()
+        torch.cuda.synchronize()
+        check(self._L.chimera_adj_segment_vjp(self._h, _ptrs(x0), dts.ctypes.data_as(C.POINTER(C.c_double)),
+                                              len(dts), _ptrs(lam)), "chimera_adj_segment_vjp")
+        return lam
+
     def close(self):
         if getattr(self, "_h", None):
             self._L.chimera_adj_destroy(self._h)
@@ -104,17 +115,11 @@ class _Evolve(torch.autograd.Function):
         n = len(dts)
         adj = AdjointStepper(nx, ny, dx, gamma, bc, scheme)
         lam = gN.contiguous().clone()
-        buf = torch.empty_like(lam)
         try:
             for seg in reversed(range(len(ckpts))):
                 s0 = seg * K
                 s1 = min(s0 + K, n)
-                states = [ckpts[seg]]
-                for s in range(s0, s1 - 1):
-                    states.append(adj.forward_step(states[-1], dts[s]))
-                for s in reversed(range(s0, s1)):
-                    adj.vjp(states[s - s0], dts[s], lam, out=buf)
-                    lam, buf = buf, lam
+                adj.segment_vjp(ckpts[seg], dts[s0:s1], lam)
         finally:
             adj.close()
         return lam, None, None, None

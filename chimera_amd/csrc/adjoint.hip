@@ -3,21 +3,22 @@
 // the conservative update, constrained transport, the outflow correction, the conserved
 // floors and the RK blends.
 //
-// Per stage, forward kernels af_* recompute the stage from its input exactly as the
-// fused kernel does (same operations in the same order) and keep what the reverse
-// needs -- primitives, face states after MOOD and the thermal check, branch bits,
-// fluxes, the pre-floor and pre-blend states.  Reverse kernels ar_* are written in
-// gather form (each thread owns one output location and collects the contributions
-// of every forward operation that read it), so there are no atomics and the
-// gradient is deterministic.  Branches (limiter cases, MOOD resets, face and
-// conserved floors, thermal resets, Riemann-solver regions and catches) are the ones
-// the forward pass took; Riemann Jacobians come from forward-mode dual numbers
-// through riemann_dual.hpp.
+// Per stage, the fused LDS-tiled stage kernel (stage_kernel.hpp, STORE instantiation)
+// recomputes the stage exactly as the forward path does and keeps what the reverse
+// needs -- primitives, face states after MOOD and the thermal check, branch bits, the
+// pre-floor and pre-blend states.  Reverse kernels ar_* are written in gather form
+// (each thread owns one output location and collects the contributions of every
+// forward operation that read it), so there are no atomics and the gradient is
+// deterministic.  Branches (limiter cases, MOOD resets, face and conserved floors,
+// thermal resets, Riemann-solver regions and catches) are the ones the forward pass
+// took.  Rusanov and HLLE are reversed by hand (riemann_adj.hpp); HLLD Jacobians come
+// from forward-mode dual numbers (riemann_dual.hpp).
 #include <vector>
 
 #include "adjoint_phys.hpp"
 #include "common.hpp"
-#include "riemann_dual.hpp"
+#include "resident.hpp"
+#include "riemann_adj.hpp"
 
 namespace chimera {
 namespace {
@@ -30,20 +31,13 @@ struct AdjGrid {
     int use_lim, mood;
 };
 
-// per-stage intermediates (device arrays of n doubles unless noted)
-struct StageBufs {
-    double *P[6];       // primitives rho, vx, vy, P, Bx, By
-    double *face[24];   // field f, side s (XL, XR, YL, YR) at 4*f + s, after thermal check
-    unsigned long long *bits;  // MOOD 4f+s | thermal 24+s | face floor 32 + 4k + s (k: rho, P)
-    double *FX[5], *FY[5];     // Mass, Momx, Momy, Energy, B flux
-    double *Up[4];      // after the flux update (before floors)
-    double *U3[4];      // after floors, CT and the outflow correction (before the blend)
-};
+using StageBufs = StageStore;
 
 struct RevBufs {
     double *lUp[4], *lBp[2], *lBc[2], *lPd[6], *lnegEz;
     double *lXL[6], *lXR[6], *lYL[6], *lYR[6], *lPr[6];
-    double *lSx[6], *lSy[6], *lP[6];
+    double *gmx[6], *gpx[6], *gmy[6], *gpy[6];  // limiter parts passed to the lo / hi neighbour
+    double *lP[6];
     double *lBcT[2];
 };
 
@@ -56,198 +50,20 @@ __device__ __forceinline__ int wrap_or_clamp(int g, int n, int periodic) {
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (n); \
          k += (size_t)gridDim.x * blockDim.x)
 
-// ---- forward recompute ------------------------------------------------------------
+// ---- forward recompute: the fused stage kernel, keeping the intermediates -----------
 
-__global__ void af_prims(AdjGrid G, const double *const *U, StageBufs S) {
-    const int nx = G.nx, ny = G.ny;
-    CELL_LOOP(k, (size_t)nx * ny) {
-        int i = (int)(k % nx), j = (int)(k / nx);
-        int im = i > 0 ? i - 1 : (G.px ? nx - 1 : 0);
-        int jm = j > 0 ? j - 1 : (G.py ? ny - 1 : 0);
-        double Bx = 0.5 * (U[4][k] + U[4][(size_t)im + (size_t)nx * j]);
-        double By = 0.5 * (U[5][k] + U[5][(size_t)i + (size_t)nx * jm]);
-        double r, u, w, p;
-        cons_to_prim(U[0][k], U[1][k], U[2][k], U[3][k], Bx, By, G.gamma, G.vol, r, u, w, p);
-        S.P[0][k] = r;
-        S.P[1][k] = u;
-        S.P[2][k] = w;
-        S.P[3][k] = p;
-        S.P[4][k] = Bx;
-        S.P[5][k] = By;
-    }
-}
+constexpr int ATX = 32, ATY = 8;
 
-template <int LIMITER>
-__device__ __forceinline__ double slope_at(const double *f, int i, int j, int dir, const AdjGrid &G,
-                                           double inv_dx, double inv_2dx) {
-    const int nx = G.nx, ny = G.ny;
-    double fm, f0, fp;
-    f0 = f[(size_t)i + (size_t)nx * j];
-    if (dir == 0) {
-        fm = f[(size_t)wrap_or_clamp(i - 1, nx, G.px) + (size_t)nx * j];
-        fp = f[(size_t)wrap_or_clamp(i + 1, nx, G.px) + (size_t)nx * j];
-    } else {
-        fm = f[(size_t)i + (size_t)nx * wrap_or_clamp(j - 1, ny, G.py)];
-        fp = f[(size_t)i + (size_t)nx * wrap_or_clamp(j + 1, ny, G.py)];
-    }
-    return G.use_lim ? limited_slope<LIMITER>(fm, f0, fp, inv_dx) : central_slope(fm, fp, inv_2dx);
-}
-
-// faces of cell (i,j): XL, XR (from i+1), YL, YR (from j+1); MOOD against the stencil
-// of (i,j); rho/P face floor; thermal check against cell (i,j).
-template <int LIMITER>
-__global__ void af_faces(AdjGrid G, StageBufs S) {
-    const int nx = G.nx, ny = G.ny;
-    const double inv_dx = 1.0 / G.dx, inv_2dx = 1.0 / (2.0 * G.dx), hdx = 0.5 * G.dx;
-    CELL_LOOP(k, (size_t)nx * ny) {
-        int i = (int)(k % nx), j = (int)(k / nx);
-        int ip = wrap_or_clamp(i + 1, nx, G.px), im = wrap_or_clamp(i - 1, nx, G.px);
-        int jp = wrap_or_clamp(j + 1, ny, G.py), jm = wrap_or_clamp(j - 1, ny, G.py);
-        bool lastx = !G.px && i == nx - 1, lasty = !G.py && j == ny - 1;
-        unsigned long long bits = 0;
-        double F[4][6], C[6];
-        for (int f = 0; f < 6; ++f) {
-            const double *a = S.P[f];
-            double c = a[k];
-            C[f] = c;
-            double sx = slope_at<LIMITER>(a, i, j, 0, G, inv_dx, inv_2dx);
-            double sy = slope_at<LIMITER>(a, i, j, 1, G, inv_dx, inv_2dx);
-            double xl = c + hdx * sx;
-            double yl = c + hdx * sy;
-            double xr = lastx ? c : a[(size_t)ip + (size_t)nx * j] - hdx * slope_at<LIMITER>(a, ip, j, 0, G, inv_dx, inv_2dx);
-            double yr = lasty ? c : a[(size_t)i + (size_t)nx * jp] - hdx * slope_at<LIMITER>(a, i, jp, 1, G, inv_dx, inv_2dx);
-            bool rp = (f == 0 || f == 3);
-            if (G.mood) {
-                double a1 = a[(size_t)ip + (size_t)nx * j], a2 = a[(size_t)im + (size_t)nx * j];
-                double a3 = a[(size_t)i + (size_t)nx * jp], a4 = a[(size_t)i + (size_t)nx * jm];
-                double lo = fmin(fmin(fmin(fmin(c, a1), a2), a3), a4);
-                double hi = fmax(fmax(fmax(fmax(c, a1), a2), a3), a4);
-                if (rp) lo = fmax(1e-9, lo);
-                if (xl < lo || xl > hi) { xl = c; bits |= 1ull << (4 * f + 0); }
-                if (xr < lo || xr > hi) { xr = c; bits |= 1ull << (4 * f + 1); }
-                if (yl < lo || yl > hi) { yl = c; bits |= 1ull << (4 * f + 2); }
-                if (yr < lo || yr > hi) { yr = c; bits |= 1ull << (4 * f + 3); }
-            }
-            if (rp) {
-                const int kk = f == 0 ? 0 : 1;
-                double v[4] = {xl, xr, yl, yr};
-                for (int s = 0; s < 4; ++s)
-                    if (1.0e-8 > v[s]) bits |= 1ull << (32 + 4 * kk + s);
-                xl = fmax(xl, 1.0e-8);
-                xr = fmax(xr, 1.0e-8);
-                yl = fmax(yl, 1.0e-8);
-                yr = fmax(yr, 1.0e-8);
-            }
-            F[0][f] = xl;
-            F[1][f] = xr;
-            F[2][f] = yl;
-            F[3][f] = yr;
-        }
-        for (int s = 0; s < 4; ++s) {
-            double p_th = F[s][3] - 0.5 * (F[s][4] * F[s][4] + F[s][5] * F[s][5]);
-            if (p_th < P_FLOOR) {
-                for (int f = 0; f < 6; ++f) F[s][f] = C[f];
-                bits |= 1ull << (24 + s);
-            }
-        }
-        for (int f = 0; f < 6; ++f)
-            for (int s = 0; s < 4; ++s) S.face[4 * f + s][k] = F[s][f];
-        S.bits[k] = bits;
-    }
-}
-
-template <int SOLVER>
-__global__ void af_fluxes(AdjGrid G, StageBufs S) {
-    CELL_LOOP(k, (size_t)G.nx * G.ny) {
-        FaceState a{S.face[0][k], S.face[4][k], S.face[8][k], S.face[12][k], S.face[16][k], S.face[20][k]};
-        FaceState b{S.face[1][k], S.face[5][k], S.face[9][k], S.face[13][k], S.face[17][k], S.face[21][k]};
-        Flux5 f = riemann<SOLVER>(a, b, G.gamma);
-        S.FX[0][k] = f.mass;
-        S.FX[1][k] = f.mom_n;
-        S.FX[2][k] = f.mom_t;
-        S.FX[3][k] = f.energy;
-        S.FX[4][k] = f.bt;
-        FaceState c{S.face[2][k], S.face[10][k], S.face[6][k], S.face[14][k], S.face[22][k], S.face[18][k]};
-        FaceState d{S.face[3][k], S.face[11][k], S.face[7][k], S.face[15][k], S.face[23][k], S.face[19][k]};
-        Flux5 h = riemann<SOLVER>(c, d, G.gamma);
-        S.FY[0][k] = h.mass;
-        S.FY[1][k] = h.mom_t;
-        S.FY[2][k] = h.mom_n;
-        S.FY[3][k] = h.energy;
-        S.FY[4][k] = h.bt;
-    }
-}
-
-// -Ez at corner (a, b) from the B fluxes (src/mhd_flux.f90:108-129)
-__device__ __forceinline__ double neg_ez_at(const StageBufs &S, const AdjGrid &G, int a, int b) {
-    const int nx = G.nx, ny = G.ny;
-    size_t k = (size_t)a + (size_t)nx * b;
-    double f0 = S.FX[4][k];
-    double up = (b < ny - 1) ? S.FX[4][k + nx] : (G.py ? S.FX[4][a] : f0);
-    double e0 = S.FY[4][k];
-    double rt = (a < nx - 1) ? S.FY[4][k + 1] : (G.px ? S.FY[4][(size_t)nx * b] : e0);
-    double Ez = 0.25 * (-f0 - up + e0 + rt);
-    return -Ez;
-}
-
-__global__ void af_update(AdjGrid G, const double *const *U, const double *const *U0,
-                          double *const *Uout, StageBufs S, int stage, double dt) {
-    const int nx = G.nx, ny = G.ny;
-    const double dtdx = dt * G.dx, inv_dx = 1.0 / G.dx, g = G.gamma, vol = G.vol;
-    CELL_LOOP(k, (size_t)nx * ny) {
-        int i = (int)(k % nx), j = (int)(k / nx);
-        bool xlo = i == 0 && !G.px, ylo = j == 0 && !G.py;
-        size_t kl = i > 0 ? k - 1 : k + nx - 1;
-        size_t kb = j > 0 ? k - nx : k + (size_t)nx * (ny - 1);
-        double V[4];
-        for (int q = 0; q < 4; ++q) {
-            double v = U[q][k];
-            if (xlo) v = v - dtdx * S.FX[q][k];
-            else v = v - dtdx * S.FX[q][k] + dtdx * S.FX[q][kl];
-            if (ylo) v = v - dtdx * S.FY[q][k];
-            else v = v - dtdx * S.FY[q][k] + dtdx * S.FY[q][kb];
-            V[q] = v;
-            S.Up[q][k] = v;
-        }
-        double Bxc = S.P[4][k], Byc = S.P[5][k];
-        conserved_floors(V[0], V[1], V[2], V[3], Bxc, Byc, vol, g);
-        double dbx, dby;
-        if (!ylo) dbx = (neg_ez_at(S, G, i, j) - neg_ez_at(S, G, i, j > 0 ? j - 1 : ny - 1)) * inv_dx;
-        else dbx = (neg_ez_at(S, G, i, 1) - neg_ez_at(S, G, i, 0)) * inv_dx;
-        if (!xlo) dby = -((neg_ez_at(S, G, i, j) - neg_ez_at(S, G, i > 0 ? i - 1 : nx - 1, j)) * inv_dx);
-        else dby = -((neg_ez_at(S, G, 1, j) - neg_ez_at(S, G, 0, j)) * inv_dx);
-        double bxn = U[4][k] + dt * dbx;
-        double byn = U[5][k] + dt * dby;
-        if (xlo || ylo) {
-            double rho = S.P[0][k], vx = S.P[1][k], vy = S.P[2][k], P = S.P[3][k];
-            double h = 0.5 * (Bxc * Bxc + Byc * Byc);
-            double en = (P - h) / (g - 1.0) + 0.5 * rho * (vx * vx + vy * vy) + h;
-            if (xlo) {
-                V[0] = V[0] + dtdx * rho * vx;
-                V[1] = V[1] + dtdx * (rho * (vx * vx) + P - Bxc * Bxc);
-                V[2] = V[2] + dtdx * (rho * vx * vy - Bxc * Byc);
-                V[3] = V[3] + dtdx * ((en + P) * vx - Bxc * (Bxc * vx + Byc * vy));
-            }
-            if (ylo) {
-                V[0] = V[0] + dtdx * rho * vy;
-                V[2] = V[2] + dtdx * (rho * (vy * vy) + P - Byc * Byc);
-                V[1] = V[1] + dtdx * (rho * vy * vx - Byc * Bxc);
-                V[3] = V[3] + dtdx * ((en + P) * vy - Byc * (Byc * vy + Bxc * vx));
-            }
-        }
-        for (int q = 0; q < 4; ++q) S.U3[q][k] = V[q];
-        if (stage > 1) {
-            const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
-            const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
-            for (int q = 0; q < 4; ++q) V[q] = c0 * U0[q][k] + c1 * V[q];
-            bxn = c0 * U0[4][k] + c1 * bxn;
-            byn = c0 * U0[5][k] + c1 * byn;
-            conserved_floors(V[0], V[1], V[2], V[3], Bxc, Byc, vol, g);
-        }
-        for (int q = 0; q < 4; ++q) Uout[q][k] = V[q];
-        Uout[4][k] = bxn;
-        Uout[5][k] = byn;
-    }
+template <int SOLVER, int LIMITER>
+__global__ __launch_bounds__(STAGE_THREADS) void adj_stage_kernel(GridView g, StageStore st,
+                                                                 int stage, int use_lim, int mood,
+                                                                 int tiles_x, int ntiles) {
+    __shared__ double lds[TileShape<ATX, ATY>::LDS_DOUBLES];
+    const int b = blockIdx.x, nb = gridDim.x;  // nb is a multiple of 8
+    const int tile = (b % 8) * (nb / 8) + b / 8;
+    if (tile >= ntiles) return;
+    stage_tile<SOLVER, LIMITER, ATX, ATY, true>(g, stage, use_lim != 0, mood != 0, tile % tiles_x,
+                                                tile / tiles_x, lds, &st);
 }
 
 // ---- reverse -------------------------------------------------------------------
@@ -417,7 +233,7 @@ __device__ __forceinline__ void face_reverse(const AdjGrid &G, const StageBufs &
     // Riemann output order mass, mom_n, mom_t, energy, bt
     double lFr[5] = {lF[0], XDIR ? lF[1] : lF[2], XDIR ? lF[2] : lF[1], lF[3], lF[4]};
     double lLr[6], lRr[6];
-    riemann_vjp<SOLVER, 1>(a, b, G.gamma, lFr, lLr, lRr);
+    riemann_rev<SOLVER>(a, b, G.gamma, lFr, lLr, lRr);
     double lL[6], lR[6];   // field order rho, vx, vy, P, Bx, By
     lL[0] = lLr[0]; lL[fu] = lLr[1]; lL[fw] = lLr[2]; lL[3] = lLr[3]; lL[fbn] = lLr[4]; lL[fbt] = lLr[5];
     lR[0] = lRr[0]; lR[fu] = lRr[1]; lR[fw] = lRr[2]; lR[3] = lRr[3]; lR[fbn] = lRr[4]; lR[fbt] = lRr[5];
@@ -433,7 +249,7 @@ __global__ __launch_bounds__(256) void ar_faces(AdjGrid G, StageBufs S, RevBufs 
     const int nx = G.nx, ny = G.ny;
     CELL_LOOP(k, (size_t)nx * ny) {
         int i = (int)(k % nx), j = (int)(k / nx);
-        unsigned long long bits = S.bits[k];
+        unsigned long long bits = S.bits_x[k] | S.bits_y[k];
         double lreset[6] = {0, 0, 0, 0, 0, 0};
         face_reverse<SOLVER, true>(G, S, R, dt, i, j, k, bits, lreset);
         face_reverse<SOLVER, false>(G, S, R, dt, i, j, k, bits, lreset);
@@ -442,66 +258,75 @@ __global__ __launch_bounds__(256) void ar_faces(AdjGrid G, StageBufs S, RevBufs 
     }
 }
 
-// lam of the slopes and of the cell values reached directly.
-__global__ void ar_slopes(AdjGrid G, RevBufs R) {
+// lam of each limited slope (from the face states built with it), its limiter gradient
+// (once per slope), the parts passed to the lo / hi neighbour (gm / gp) and the cell's
+// own part of lam of its primitives.
+template <int LIMITER>
+__global__ void ar_slopes(AdjGrid G, StageStore S, RevBufs R) {
     const int nx = G.nx, ny = G.ny;
-    const double hdx = 0.5 * G.dx;
+    const double hdx = 0.5 * G.dx, inv_dx = 1.0 / G.dx, inv_2dx = 1.0 / (2.0 * G.dx);
     CELL_LOOP(k, (size_t)nx * ny) {
         int i = (int)(k % nx), j = (int)(k / nx);
         bool hl = i > 0 || G.px, hb = j > 0 || G.py;   // a face on the lo side exists
         size_t kl = i > 0 ? k - 1 : k + nx - 1;
         size_t kb = j > 0 ? k - nx : k + (size_t)nx * (ny - 1);
+        // neighbours the slopes of (i, j) read: periodic wrap or zero-gradient clamp
+        size_t kxm = (size_t)wrap_or_clamp(i - 1, nx, G.px) + (size_t)nx * j;
+        size_t kxp = (size_t)wrap_or_clamp(i + 1, nx, G.px) + (size_t)nx * j;
+        size_t kym = (size_t)i + (size_t)nx * wrap_or_clamp(j - 1, ny, G.py);
+        size_t kyp = (size_t)i + (size_t)nx * wrap_or_clamp(j + 1, ny, G.py);
         for (int f = 0; f < 6; ++f) {
+            const double *a = S.P[f];
             double xr = hl ? R.lXR[f][kl] : 0.0;   // the right state of face i-1/2 came from (i,j)
             double yr = hb ? R.lYR[f][kb] : 0.0;
-            R.lSx[f][k] = hdx * (R.lXL[f][k] - xr);
-            R.lSy[f][k] = hdx * (R.lYL[f][k] - yr);
-            R.lP[f][k] = R.lXL[f][k] + xr + R.lYL[f][k] + yr + R.lPr[f][k] + R.lPd[f][k];
+            double lSx = hdx * (R.lXL[f][k] - xr);
+            double lSy = hdx * (R.lYL[f][k] - yr);
+            double lp = R.lXL[f][k] + xr + R.lYL[f][k] + yr + R.lPr[f][k] + R.lPd[f][k];
+            double dm, d0, dp;
+            // x slope
+            if (G.use_lim) limited_slope_grad<LIMITER>(a[kxm], a[k], a[kxp], inv_dx, dm, d0, dp);
+            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
+            double gm = lSx * dm, gp = lSx * dp;
+            lp += lSx * d0;
+            if (kxm == k) { lp += gm; gm = 0.0; }   // clamped at a non-periodic edge
+            if (kxp == k) { lp += gp; gp = 0.0; }
+            R.gmx[f][k] = gm;
+            R.gpx[f][k] = gp;
+            // y slope
+            if (G.use_lim) limited_slope_grad<LIMITER>(a[kym], a[k], a[kyp], inv_dx, dm, d0, dp);
+            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
+            gm = lSy * dm;
+            gp = lSy * dp;
+            lp += lSy * d0;
+            if (kym == k) { lp += gm; gm = 0.0; }
+            if (kyp == k) { lp += gp; gp = 0.0; }
+            R.gmy[f][k] = gm;
+            R.gpy[f][k] = gp;
+            R.lP[f][k] = lp;
         }
     }
 }
 
-// Reverse of the limiter (gather over the three slopes that read this cell), then of
+// lam of the primitives (own part + the neighbours' slope parts), then the reverse of
 // cons_to_prim; writes lam of the conserved stage input and the total lam of cell B.
-template <int LIMITER>
-__global__ void ar_prims(AdjGrid G, const double *const *U, StageBufs S, RevBufs R,
+__global__ void ar_prims(AdjGrid G, const double *const *U, StageStore S, RevBufs R,
                          double *const *lin) {
     const int nx = G.nx, ny = G.ny;
-    const double inv_dx = 1.0 / G.dx, inv_2dx = 1.0 / (2.0 * G.dx);
     CELL_LOOP(k, (size_t)nx * ny) {
         int i = (int)(k % nx), j = (int)(k / nx);
+        bool hl = i > 0 || G.px, hr = i < nx - 1 || G.px;
+        bool hb = j > 0 || G.py, ht = j < ny - 1 || G.py;
+        size_t kl = i > 0 ? k - 1 : k + nx - 1;
+        size_t kr = i < nx - 1 ? k + 1 : k - (nx - 1);
+        size_t kb = j > 0 ? k - nx : k + (size_t)nx * (ny - 1);
+        size_t kt = j < ny - 1 ? k + nx : k - (size_t)nx * (ny - 1);
         double lp6[6];
         for (int f = 0; f < 6; ++f) {
             double acc = R.lP[f][k];
-            const double *a = S.P[f];
-            for (int dir = 0; dir < 2; ++dir) {
-                const int n = dir == 0 ? nx : ny, per = dir == 0 ? G.px : G.py;
-                const int c = dir == 0 ? i : j;
-                const double *ls = dir == 0 ? R.lSx[f] : R.lSy[f];
-                for (int o = -1; o <= 1; ++o) {
-                    int kc = c + o;
-                    if (per) kc = (kc + n) % n;
-                    else if (kc < 0 || kc >= n) continue;
-                    if (o != 0 && kc == c) continue;   // tiny periodic axes
-                    int km = wrap_or_clamp(kc - 1, n, per), kp = wrap_or_clamp(kc + 1, n, per);
-                    auto at = [&](int t) {
-                        return dir == 0 ? a[(size_t)t + (size_t)nx * j] : a[(size_t)i + (size_t)nx * t];
-                    };
-                    size_t ks = dir == 0 ? (size_t)kc + (size_t)nx * j : (size_t)i + (size_t)nx * kc;
-                    double lam = ls[ks];
-                    if (lam == 0.0) continue;
-                    double dm, d0, dp;
-                    if (G.use_lim) {
-                        limited_slope_grad<LIMITER>(at(km), at(kc), at(kp), inv_dx, dm, d0, dp);
-                    } else {
-                        dm = -inv_2dx;
-                        d0 = 0.0;
-                        dp = inv_2dx;
-                    }
-                    double w = (km == c ? dm : 0.0) + (kc == c ? d0 : 0.0) + (kp == c ? dp : 0.0);
-                    acc += lam * w;
-                }
-            }
+            if (hl) acc += R.gpx[f][kl];
+            if (hr) acc += R.gmx[f][kr];
+            if (hb) acc += R.gpy[f][kb];
+            if (ht) acc += R.gmy[f][kt];
             lp6[f] = acc;
         }
         double Bx = S.P[4][k], By = S.P[5][k];
@@ -563,23 +388,35 @@ struct chimera_adj {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     size_t n = 0;
+    int tiles_x = 0, ntiles = 0, grid = 0;   // STORE stage kernel launch shape
     double *arena = nullptr;
     StageBufs S[3];
     RevBufs R;
-    Ptr6 stage_state[3];   // U^n (copied), U^(1), U^(2)
-    Ptr6 lam_a, lam_b, lam_u0, scratch_out;
-    // device-side pointer tables (kernels take double *const *)
+    Ptr6 stage_state[3];   // U^n (copied by the one-step calls), U^(1), U^(2)
+    Ptr6 lam_a, lam_b, lam_c, lam_u0, scratch_out;
+    // device-side pointer tables (reverse kernels take double *const *)
     double **dptr = nullptr;
+    // dt of the steps being processed (device; the stage kernel reads *dt)
+    double *dt_dev = nullptr;
+    int dt_cap = 0;
+    // checkpoint recomputation: fused forward and the recomputed states
+    chimera_ctx_t *fwd = nullptr;
+    double *traj = nullptr;       // traj_cap x 6 fields
+    double **traj_tab = nullptr;  // device tables, 6 pointers per slot
+    int traj_cap = 0;
 };
 
 namespace {
 
-// device pointer tables: [0] stage_state[0..2], [3] lam_a, [4] lam_b, [5] lam_u0,
-// [6] scratch_out, [7] caller input, [8] caller output, [9] caller lam in, [10] caller lam out
+// device pointer tables: [0..2] stage_state, [3] lam_a, [4] lam_b, [5] lam_u0,
+// [6] scratch_out, [7] caller input, [8] caller output, [9] caller lam in,
+// [10] caller lam out, [11] lam_c
 enum { TB_S0 = 0, TB_S1 = 1, TB_S2 = 2, TB_LA = 3, TB_LB = 4, TB_LU0 = 5, TB_SCR = 6, TB_IN = 7,
-       TB_OUT = 8, TB_LIN = 9, TB_LOUT = 10, TB_COUNT = 11 };
+       TB_OUT = 8, TB_LIN = 9, TB_LOUT = 10, TB_LC = 11, TB_COUNT = 12 };
 
-double *const *table(chimera_adj *a, int t) { return a->dptr + 6 * t; }
+using Tab = double *const *;
+
+Tab table(chimera_adj *a, int t) { return a->dptr + 6 * t; }
 
 // The caller's pointer arrays may be temporaries: copy them synchronously (after
 // the work already queued on the stream).
@@ -589,46 +426,125 @@ int set_table(chimera_adj *a, int t, const double *const *p) {
     return CHIMERA_OK;
 }
 
-int run_forward_stage(chimera_adj *a, int stage, int t_in, int t_u0, int t_out, double dt) {
-    StageBufs &S = a->S[stage - 1];
-    const int nb = nblocks(a->n);
-    hipLaunchKernelGGL(af_prims, dim3(nb), dim3(BS), 0, a->stream, a->G, table(a, t_in), S);
-    if (a->sch.slope_limiter == LIM_MC)
-        hipLaunchKernelGGL(af_faces<LIM_MC>, dim3(nb), dim3(BS), 0, a->stream, a->G, S);
-    else
-        hipLaunchKernelGGL(af_faces<LIM_VAN_LEER>, dim3(nb), dim3(BS), 0, a->stream, a->G, S);
-    switch (a->sch.riemann_solver) {
-    case RUSANOV: hipLaunchKernelGGL(af_fluxes<RUSANOV>, dim3(nb), dim3(BS), 0, a->stream, a->G, S); break;
-    case HLLE: hipLaunchKernelGGL(af_fluxes<HLLE>, dim3(nb), dim3(BS), 0, a->stream, a->G, S); break;
-    default: hipLaunchKernelGGL(af_fluxes<HLLD>, dim3(nb), dim3(BS), 0, a->stream, a->G, S); break;
+Ptr6 ptr6(const double *const *p) {
+    Ptr6 r;
+    for (int q = 0; q < 6; ++q) r.p[q] = const_cast<double *>(p[q]);
+    return r;
+}
+
+// dts of the steps to come, on the device (synchronous upload)
+int set_dts(chimera_adj *a, const double *dts, int k) {
+    CH_HIP(hipStreamSynchronize(a->stream));
+    if (k > a->dt_cap) {
+        (void)hipFree(a->dt_dev);
+        a->dt_dev = nullptr;
+        a->dt_cap = 0;
+        CH_HIP(hipMalloc(&a->dt_dev, (size_t)k * sizeof(double)));
+        a->dt_cap = k;
     }
-    hipLaunchKernelGGL(af_update, dim3(nb), dim3(BS), 0, a->stream, a->G, table(a, t_in),
-                       table(a, t_u0), table(a, t_out), S, stage, dt);
+    CH_HIP(hipMemcpy(a->dt_dev, dts, (size_t)k * sizeof(double), hipMemcpyHostToDevice));
+    return CHIMERA_OK;
+}
+
+using AdjStageFn = void (*)(GridView, StageStore, int, int, int, int, int);
+
+AdjStageFn adj_stage_fn(int solver, int limiter) {
+    static const AdjStageFn t[3][2] = {
+        {adj_stage_kernel<RUSANOV, LIM_VAN_LEER>, adj_stage_kernel<RUSANOV, LIM_MC>},
+        {adj_stage_kernel<HLLE, LIM_VAN_LEER>, adj_stage_kernel<HLLE, LIM_MC>},
+        {adj_stage_kernel<HLLD, LIM_VAN_LEER>, adj_stage_kernel<HLLD, LIM_MC>}};
+    return t[solver - 1][limiter - 1];
+}
+
+// One stage through the fused kernel, keeping the intermediates in S[stage - 1].
+int run_forward_stage(chimera_adj *a, int stage, const Ptr6 &in, const Ptr6 &u0, const Ptr6 &out,
+                      const double *dt_dev) {
+    GridView g{};
+    for (int q = 0; q < 6; ++q) {
+        g.uin[q] = in.p[q];
+        g.u0[q] = u0.p[q];
+        g.uout[q] = out.p[q];
+    }
+    g.dt = dt_dev;
+    g.mask = nullptr;
+    g.nx = a->G.nx;
+    g.ny = a->G.ny;
+    g.px = a->G.px;
+    g.py = a->G.py;
+    g.dx = a->G.dx;
+    g.vol = a->G.vol;
+    g.gamma = a->G.gamma;
+    hipLaunchKernelGGL(adj_stage_fn(a->sch.riemann_solver, a->sch.slope_limiter), dim3(a->grid),
+                       dim3(STAGE_THREADS), 0, a->stream, g, a->S[stage - 1], stage, a->G.use_lim,
+                       a->G.mood, a->tiles_x, a->ntiles);
     return launch_check("adjoint forward stage");
 }
 
-int run_reverse_stage(chimera_adj *a, int stage, int t_in, int t_u0, int t_lout, int t_lin,
-                      double dt) {
+int run_reverse_stage(chimera_adj *a, int stage, Tab tin, Tab tu0, Tab tlout, Tab tlin, double dt) {
     StageBufs &S = a->S[stage - 1];
     RevBufs &R = a->R;
     const int nb = nblocks(a->n);
-    hipLaunchKernelGGL(ar_cell_tail, dim3(nb), dim3(BS), 0, a->stream, a->G, table(a, t_u0), S, R,
-                       table(a, t_lout), table(a, t_lin), table(a, TB_LU0), stage, dt);
+    hipLaunchKernelGGL(ar_cell_tail, dim3(nb), dim3(BS), 0, a->stream, a->G, tu0, S, R, tlout, tlin,
+                       table(a, TB_LU0), stage, dt);
     hipLaunchKernelGGL(ar_corner, dim3(nb), dim3(BS), 0, a->stream, a->G, R, dt);
     switch (a->sch.riemann_solver) {
     case RUSANOV: hipLaunchKernelGGL(ar_faces<RUSANOV>, dim3(nb), dim3(BS), 0, a->stream, a->G, S, R, dt); break;
     case HLLE: hipLaunchKernelGGL(ar_faces<HLLE>, dim3(nb), dim3(BS), 0, a->stream, a->G, S, R, dt); break;
     default: hipLaunchKernelGGL(ar_faces<HLLD>, dim3(nb), dim3(BS), 0, a->stream, a->G, S, R, dt); break;
     }
-    hipLaunchKernelGGL(ar_slopes, dim3(nb), dim3(BS), 0, a->stream, a->G, R);
     if (a->sch.slope_limiter == LIM_MC)
-        hipLaunchKernelGGL(ar_prims<LIM_MC>, dim3(nb), dim3(BS), 0, a->stream, a->G, table(a, t_in), S, R,
-                           table(a, t_lin));
+        hipLaunchKernelGGL(ar_slopes<LIM_MC>, dim3(nb), dim3(BS), 0, a->stream, a->G, S, R);
     else
-        hipLaunchKernelGGL(ar_prims<LIM_VAN_LEER>, dim3(nb), dim3(BS), 0, a->stream, a->G,
-                           table(a, t_in), S, R, table(a, t_lin));
-    hipLaunchKernelGGL(ar_bavg, dim3(nb), dim3(BS), 0, a->stream, a->G, R, table(a, t_lin));
+        hipLaunchKernelGGL(ar_slopes<LIM_VAN_LEER>, dim3(nb), dim3(BS), 0, a->stream, a->G, S, R);
+    hipLaunchKernelGGL(ar_prims, dim3(nb), dim3(BS), 0, a->stream, a->G, tin, S, R, tlin);
+    hipLaunchKernelGGL(ar_bavg, dim3(nb), dim3(BS), 0, a->stream, a->G, R, tlin);
     return launch_check("adjoint reverse stage");
+}
+
+// lam_in = (dU^{n+1}/dU^n)^T lam_out for one step at U^n (host pointers hs, device
+// table ts), dt = *dt_dev; asynchronous.  tlout and tlin must differ; neither may be
+// LA, LB or LU0.
+int vjp_core(chimera_adj *a, const Ptr6 &hs, Tab ts, const double *dt_dev, double dt, Tab tlout,
+             Tab tlin) {
+    const int nb = nblocks(a->n);
+    // forward: keep every stage's intermediates
+    CH_TRY(run_forward_stage(a, 1, hs, hs, a->stage_state[1], dt_dev));
+    CH_TRY(run_forward_stage(a, 2, a->stage_state[1], hs, a->stage_state[2], dt_dev));
+    CH_TRY(run_forward_stage(a, 3, a->stage_state[2], hs, a->scratch_out, dt_dev));
+    hipLaunchKernelGGL(zero6_kernel, dim3(nb), dim3(BS), 0, a->stream, table(a, TB_LU0), a->n);
+    // reverse: stage 3 (lam_out -> LB), stage 2 (LB -> LA), stage 1 (LA -> lam_in)
+    CH_TRY(run_reverse_stage(a, 3, table(a, TB_S2), ts, tlout, table(a, TB_LB), dt));
+    CH_TRY(run_reverse_stage(a, 2, table(a, TB_S1), ts, table(a, TB_LB), table(a, TB_LA), dt));
+    CH_TRY(run_reverse_stage(a, 1, ts, ts, table(a, TB_LA), tlin, dt));
+    hipLaunchKernelGGL(add_kernel, dim3(nb), dim3(BS), 0, a->stream, tlin,
+                       (const double *const *)table(a, TB_LU0), a->n);
+    return launch_check("adjoint accumulate");
+}
+
+int ensure_traj(chimera_adj *a, int slots) {
+    if (slots <= a->traj_cap) return CHIMERA_OK;
+    CH_HIP(hipStreamSynchronize(a->stream));
+    (void)hipFree(a->traj);
+    (void)hipFree(a->traj_tab);
+    a->traj = nullptr;
+    a->traj_tab = nullptr;
+    a->traj_cap = 0;
+    if (hipMalloc(&a->traj, (size_t)slots * 6 * a->n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&a->traj_tab, (size_t)slots * 6 * sizeof(double *)) != hipSuccess) {
+        set_error("adjoint trajectory storage (%d states of %zu cells) failed", slots, a->n);
+        return CHIMERA_ENOMEM;
+    }
+    std::vector<double *> h((size_t)slots * 6);
+    for (size_t k = 0; k < h.size(); ++k) h[k] = a->traj + k * a->n;
+    CH_HIP(hipMemcpy(a->traj_tab, h.data(), h.size() * sizeof(double *), hipMemcpyHostToDevice));
+    a->traj_cap = slots;
+    return CHIMERA_OK;
+}
+
+Ptr6 traj_slot(chimera_adj *a, int s) {
+    Ptr6 r;
+    for (int q = 0; q < 6; ++q) r.p[q] = a->traj + (6 * (size_t)s + q) * a->n;
+    return r;
 }
 
 }  // namespace
@@ -636,8 +552,12 @@ int run_reverse_stage(chimera_adj *a, int stage, int t_in, int t_u0, int t_lout,
 extern "C" int chimera_adj_destroy(chimera_adj_t *a) {
     if (!a) return CHIMERA_OK;
     if (a->stream) (void)hipStreamSynchronize(a->stream);
+    if (a->fwd) chimera_ctx_destroy(a->fwd);
     (void)hipFree(a->arena);
     (void)hipFree(a->dptr);
+    (void)hipFree(a->dt_dev);
+    (void)hipFree(a->traj);
+    (void)hipFree(a->traj_tab);
     if (a->own_stream && a->stream) (void)hipStreamDestroy(a->stream);
     delete a;
     return CHIMERA_OK;
@@ -657,6 +577,9 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
                    scheme->fallback_2_mood};
     a->sch = *scheme;
     a->n = (size_t)nx * ny;
+    a->tiles_x = (nx + ATX - 1) / ATX;
+    a->ntiles = a->tiles_x * ((ny + ATY - 1) / ATY);
+    a->grid = (a->ntiles + 7) / 8 * 8;
     if (stream) {
         a->stream = (hipStream_t)stream;
     } else {
@@ -667,9 +590,9 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
         }
         a->own_stream = true;
     }
-    // per stage: 6 P + 24 faces + 1 bits + 10 fluxes + 4 Up + 4 U3 = 49
-    // reverse: 4 + 2 + 2 + 6 + 1 + 30 + 12 + 6 + 2 = 65; states 3 x 6; lambdas 4 x 6
-    const size_t per_stage = 49, rev = 65, extra = 18 + 24;
+    // per stage: 6 P + 24 faces + 2 bits + 4 Up + 4 U3 = 40
+    // reverse: 4 + 2 + 2 + 6 + 1 + 24 + 6 + 24 + 6 + 2 = 77; states 3 x 6; lambdas 5 x 6
+    const size_t per_stage = 40, rev = 77, extra = 18 + 30;
     const size_t total = 3 * per_stage + rev + extra;
     if (hipMalloc(&a->arena, total * a->n * sizeof(double)) != hipSuccess ||
         hipMalloc(&a->dptr, 6 * TB_COUNT * sizeof(double *)) != hipSuccess) {
@@ -687,9 +610,8 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
         StageBufs &S = a->S[s];
         for (auto &x : S.P) x = take();
         for (auto &x : S.face) x = take();
-        S.bits = reinterpret_cast<unsigned long long *>(take());
-        for (auto &x : S.FX) x = take();
-        for (auto &x : S.FY) x = take();
+        S.bits_x = reinterpret_cast<unsigned long long *>(take());
+        S.bits_y = reinterpret_cast<unsigned long long *>(take());
         for (auto &x : S.Up) x = take();
         for (auto &x : S.U3) x = take();
     }
@@ -704,22 +626,26 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
     for (auto &x : R.lYL) x = take();
     for (auto &x : R.lYR) x = take();
     for (auto &x : R.lPr) x = take();
-    for (auto &x : R.lSx) x = take();
-    for (auto &x : R.lSy) x = take();
+    for (auto &x : R.gmx) x = take();
+    for (auto &x : R.gpx) x = take();
+    for (auto &x : R.gmy) x = take();
+    for (auto &x : R.gpy) x = take();
     for (auto &x : R.lP) x = take();
     for (auto &x : R.lBcT) x = take();
     for (auto &st : a->stage_state)
         for (auto &x : st.p) x = take();
     for (auto &x : a->lam_a.p) x = take();
     for (auto &x : a->lam_b.p) x = take();
+    for (auto &x : a->lam_c.p) x = take();
     for (auto &x : a->lam_u0.p) x = take();
     for (auto &x : a->scratch_out.p) x = take();
-    int rc = CHIMERA_OK;
+    int rc = chimera_ctx_create(&a->fwd, nx, ny, dx, gamma, bc, scheme, a->stream);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_S0, a->stage_state[0].p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_S1, a->stage_state[1].p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_S2, a->stage_state[2].p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_LA, a->lam_a.p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_LB, a->lam_b.p);
+    if (rc == CHIMERA_OK) rc = set_table(a, TB_LC, a->lam_c.p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_LU0, a->lam_u0.p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_SCR, a->scratch_out.p);
     if (rc == CHIMERA_OK && hipStreamSynchronize(a->stream) != hipSuccess) rc = CHIMERA_EHIP;
@@ -731,18 +657,21 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
     return CHIMERA_OK;
 }
 
-// One SSP-RK3 step with a given dt through the adjoint's forward kernels
-// (device pointers).
+// One SSP-RK3 step with a given dt through the adjoint's forward (the STORE
+// instantiation of the fused stage kernel); device pointers.
 extern "C" int chimera_adj_forward_step(chimera_adj_t *a, const double *const *state_in,
                                         double dt, double *const *state_out) {
     CH_REQUIRE(a && state_in && state_out, "NULL argument");
+    CH_REQUIRE(dt > 0.0, "dt must be > 0");
+    CH_TRY(set_dts(a, &dt, 1));
     size_t bytes = a->n * sizeof(double);
     for (int q = 0; q < 6; ++q)
         CH_HIP(hipMemcpyAsync(a->stage_state[0].p[q], state_in[q], bytes, hipMemcpyDefault, a->stream));
-    CH_TRY(set_table(a, TB_OUT, (const double *const *)state_out));
-    CH_TRY(run_forward_stage(a, 1, TB_S0, TB_S0, TB_S1, dt));
-    CH_TRY(run_forward_stage(a, 2, TB_S1, TB_S0, TB_S2, dt));
-    CH_TRY(run_forward_stage(a, 3, TB_S2, TB_S0, TB_OUT, dt));
+    Ptr6 out = ptr6((const double *const *)state_out);
+    const Ptr6 &s0 = a->stage_state[0];
+    CH_TRY(run_forward_stage(a, 1, s0, s0, a->stage_state[1], a->dt_dev));
+    CH_TRY(run_forward_stage(a, 2, a->stage_state[1], s0, a->stage_state[2], a->dt_dev));
+    CH_TRY(run_forward_stage(a, 3, a->stage_state[2], s0, out, a->dt_dev));
     CH_HIP(hipStreamSynchronize(a->stream));
     return CHIMERA_OK;
 }
@@ -751,27 +680,53 @@ extern "C" int chimera_adj_forward_step(chimera_adj_t *a, const double *const *s
 extern "C" int chimera_adj_step_vjp(chimera_adj_t *a, const double *const *state_n, double dt,
                                     const double *const *lam_out, double *const *lam_in) {
     CH_REQUIRE(a && state_n && lam_out && lam_in, "NULL argument");
+    CH_REQUIRE(dt > 0.0, "dt must be > 0");
+    CH_TRY(set_dts(a, &dt, 1));
     size_t bytes = a->n * sizeof(double);
     for (int q = 0; q < 6; ++q) {
         CH_HIP(hipMemcpyAsync(a->stage_state[0].p[q], state_n[q], bytes, hipMemcpyDefault, a->stream));
-        CH_HIP(hipMemcpyAsync(a->lam_a.p[q], lam_out[q], bytes, hipMemcpyDefault, a->stream));
+        CH_HIP(hipMemcpyAsync(a->lam_c.p[q], lam_out[q], bytes, hipMemcpyDefault, a->stream));
     }
     CH_TRY(set_table(a, TB_LIN, (const double *const *)lam_in));
-    const int nb = nblocks(a->n);
-    // forward: keep every stage's intermediates
-    CH_TRY(run_forward_stage(a, 1, TB_S0, TB_S0, TB_S1, dt));
-    CH_TRY(run_forward_stage(a, 2, TB_S1, TB_S0, TB_S2, dt));
-    CH_TRY(run_forward_stage(a, 3, TB_S2, TB_S0, TB_SCR, dt));
-    hipLaunchKernelGGL(zero6_kernel, dim3(nb), dim3(BS), 0, a->stream, table(a, TB_LU0), a->n);
-    // reverse: stage 3 (lam_a -> lam_b), stage 2 (lam_b -> lam_a), stage 1 (lam_a -> lam_b)
-    CH_TRY(run_reverse_stage(a, 3, TB_S2, TB_S0, TB_LA, TB_LB, dt));
-    CH_TRY(run_reverse_stage(a, 2, TB_S1, TB_S0, TB_LB, TB_LA, dt));
-    CH_TRY(run_reverse_stage(a, 1, TB_S0, TB_S0, TB_LA, TB_LB, dt));
-    hipLaunchKernelGGL(add_kernel, dim3(nb), dim3(BS), 0, a->stream, table(a, TB_LB),
-                       (const double *const *)table(a, TB_LU0), a->n);
-    CH_TRY(launch_check("adjoint accumulate"));
-    for (int q = 0; q < 6; ++q)
-        CH_HIP(hipMemcpyAsync(lam_in[q], a->lam_b.p[q], bytes, hipMemcpyDefault, a->stream));
+    CH_TRY(vjp_core(a, a->stage_state[0], table(a, TB_S0), a->dt_dev, dt, table(a, TB_LC),
+                    table(a, TB_LIN)));
+    CH_HIP(hipStreamSynchronize(a->stream));
+    return CHIMERA_OK;
+}
+
+extern "C" int chimera_adj_segment_vjp(chimera_adj_t *a, const double *const *state0,
+                                       const double *dt_in, int nsteps, double *const *lam) {
+    CH_REQUIRE(a && state0 && dt_in && lam, "NULL argument");
+    CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
+    for (int s = 0; s < nsteps; ++s)
+        CH_REQUIRE(dt_in[s] > 0.0, "dt_in[%d] = %g is not positive", s, dt_in[s]);
+    // slots 1..nsteps-1 hold U^1..U^{nsteps-1}; U^0 is the caller's state0
+    CH_TRY(ensure_traj(a, nsteps));
+    CH_TRY(set_dts(a, dt_in, nsteps));
+    CH_TRY(set_table(a, TB_IN, state0));
+    CH_TRY(set_table(a, TB_LIN, (const double *const *)lam));
+    if (nsteps > 1) {
+        CH_TRY(chimera_ctx_upload(a->fwd, state0));
+        std::vector<double *> traj((size_t)(nsteps - 1) * 6);
+        for (size_t k = 0; k < traj.size(); ++k) traj[k] = a->traj + (6 + k) * a->n;
+        CH_TRY(chimera_ctx_steps_record(a->fwd, nsteps - 1, dt_in, traj.data()));
+    }
+    // lam ping-pongs between the caller's arrays and lam_c
+    const Ptr6 h0 = ptr6(state0);
+    Tab cur = table(a, TB_LIN), other = table(a, TB_LC);
+    for (int s = nsteps - 1; s >= 0; --s) {
+        const Ptr6 hs = s == 0 ? h0 : traj_slot(a, s);
+        Tab ts = s == 0 ? table(a, TB_IN) : (Tab)(a->traj_tab + 6 * (size_t)s);
+        CH_TRY(vjp_core(a, hs, ts, a->dt_dev + s, dt_in[s], cur, other));
+        Tab t = cur;
+        cur = other;
+        other = t;
+    }
+    if (cur != table(a, TB_LIN)) {
+        size_t bytes = a->n * sizeof(double);
+        for (int q = 0; q < 6; ++q)
+            CH_HIP(hipMemcpyAsync(lam[q], a->lam_c.p[q], bytes, hipMemcpyDeviceToDevice, a->stream));
+    }
     CH_HIP(hipStreamSynchronize(a->stream));
     return CHIMERA_OK;
 }

@@ -111,9 +111,10 @@ __device__ __forceinline__ void limited_slope_grad(double fm, double f0, double 
     if (prod <= 0.0 || fabs(den) < 1e-12) return;
     double sL, sR;  // d slope / d dfL, d slope / d dfR
     if (LIMITER == LIM_VAN_LEER) {
-        double den2 = den * den;
-        sL = 2.0 * dfR * dfR / den2;
-        sR = 2.0 * dfL * dfL / den2;
+        const double r = 1.0 / den;
+        const double a = dfR * r, b = dfL * r;
+        sL = 2.0 * a * a;
+        sR = 2.0 * b * b;
     } else {
         double m1 = 2.0 * fabs(dfL), m2 = 2.0 * fabs(dfR), m3 = 0.5 * fabs(den);
         double m12 = (m2 < m1) ? m2 : m1;  // fmin keeps the first on ties

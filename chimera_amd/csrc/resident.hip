@@ -2,6 +2,7 @@
 // in HBM) and the small device kernels around the fused stage kernel: CFL
 // max-reduction, dt policy, step bookkeeping and the div B monitor.
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -467,34 +468,59 @@ extern "C" int chimera_ctx_download(chimera_ctx_t *c, double *const *state) {
     return CHIMERA_OK;
 }
 
+// Upload a prescribed dt sequence and restart the device step counter at 0.
+static int prepare_prescribed(chimera_ctx *c, int nsteps, const double *dt_in) {
+    CH_REQUIRE(dt_in, "CHIMERA_DT_PRESCRIBED needs dt_in");
+    if (nsteps > c->dt_cap) {
+        (void)hipFree(c->dt_seq);
+        c->dt_seq = nullptr;
+        c->dt_cap = 0;
+        // graphs captured with the old pointer are stale
+        for (auto &row : c->gexec)
+            if (row[CHIMERA_DT_PRESCRIBED]) {
+                (void)hipGraphExecDestroy(row[CHIMERA_DT_PRESCRIBED]);
+                row[CHIMERA_DT_PRESCRIBED] = nullptr;
+            }
+        CH_HIP(hipMalloc(&c->dt_seq, (size_t)nsteps * sizeof(double)));
+        c->dt_cap = nsteps;
+    }
+    CH_HIP(hipMemcpyAsync(c->dt_seq, dt_in, (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+    StepState h;
+    CH_HIP(hipMemcpyAsync(&h, c->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    CH_HIP(hipStreamSynchronize(c->stream));
+    return launch_set_state(c->st, h.t, h.out_count, 0, c->stream);
+}
+
+// One full step (graph launch or plain launches); flips the current state buffer.
+static int launch_one_step(chimera_ctx *c, int dt_mode, bool use_graph, bool timed) {
+    if (use_graph) {
+        hipGraphExec_t &ge = c->gexec[c->cur][dt_mode];
+        if (!ge) {
+            hipGraph_t graph;
+            CH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_step(c, dt_mode, false);
+            hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (rc != CHIMERA_OK) return rc;
+            CH_HIP(e);
+            CH_HIP(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+        }
+        CH_HIP(hipGraphLaunch(ge, c->stream));
+    } else {
+        CH_TRY(enqueue_step(c, dt_mode, timed));
+    }
+    c->cur = 1 - c->cur;
+    return CHIMERA_OK;
+}
+
 extern "C" int chimera_ctx_steps(chimera_ctx_t *c, int nsteps, int dt_mode, const double *dt_in,
                                  int use_graph) {
     CH_REQUIRE(c, "ctx is NULL");
     CH_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
     CH_REQUIRE(dt_mode >= 0 && dt_mode <= 2, "dt_mode must be 0, 1 or 2");
     if (nsteps == 0) return CHIMERA_OK;
-    if (dt_mode == CHIMERA_DT_PRESCRIBED) {
-        CH_REQUIRE(dt_in, "CHIMERA_DT_PRESCRIBED needs dt_in");
-        if (nsteps > c->dt_cap) {
-            (void)hipFree(c->dt_seq);
-            c->dt_seq = nullptr;
-            c->dt_cap = 0;
-            // graphs captured with the old pointer are stale
-            for (auto &row : c->gexec)
-                if (row[CHIMERA_DT_PRESCRIBED]) {
-                    (void)hipGraphExecDestroy(row[CHIMERA_DT_PRESCRIBED]);
-                    row[CHIMERA_DT_PRESCRIBED] = nullptr;
-                }
-            CH_HIP(hipMalloc(&c->dt_seq, (size_t)nsteps * sizeof(double)));
-            c->dt_cap = nsteps;
-        }
-        CH_HIP(hipMemcpyAsync(c->dt_seq, dt_in, (size_t)nsteps * sizeof(double),
-                              hipMemcpyHostToDevice, c->stream));
-        StepState h;
-        CH_HIP(hipMemcpyAsync(&h, c->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-        CH_HIP(hipStreamSynchronize(c->stream));
-        CH_TRY(launch_set_state(c->st, h.t, h.out_count, 0, c->stream));
-    }
+    if (dt_mode == CHIMERA_DT_PRESCRIBED) CH_TRY(prepare_prescribed(c, nsteps, dt_in));
     const bool timed = c->timing && !use_graph;
     if (timed && c->ev_used + 6 * (size_t)nsteps > c->ev.size()) {
         // grow the pool before the launches so event creation stays out of the loop
@@ -505,24 +531,38 @@ extern "C" int chimera_ctx_steps(chimera_ctx_t *c, int nsteps, int dt_mode, cons
             c->ev.push_back(e);
         }
     }
+    for (int s = 0; s < nsteps; ++s) CH_TRY(launch_one_step(c, dt_mode, use_graph != 0, timed));
+    return CHIMERA_OK;
+}
+
+namespace chimera {
+struct Fields6 {
+    double *p[6];
+};
+__global__ void copy6_kernel(Fields6 src, Fields6 dst, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x)
+        for (int q = 0; q < 6; ++q) dst.p[q][k] = src.p[q][k];
+}
+}  // namespace chimera
+
+extern "C" int chimera_ctx_steps_record(chimera_ctx_t *c, int nsteps, const double *dt_in,
+                                        double *const *traj) {
+    CH_REQUIRE(c && traj, "NULL argument");
+    CH_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+    if (nsteps == 0) return CHIMERA_OK;
+    CH_TRY(prepare_prescribed(c, nsteps, dt_in));
+    const size_t n = (size_t)c->nx * c->ny;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
     for (int s = 0; s < nsteps; ++s) {
-        if (use_graph) {
-            hipGraphExec_t &ge = c->gexec[c->cur][dt_mode];
-            if (!ge) {
-                hipGraph_t graph;
-                CH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                int rc = enqueue_step(c, dt_mode, false);
-                hipError_t e = hipStreamEndCapture(c->stream, &graph);
-                if (rc != CHIMERA_OK) return rc;
-                CH_HIP(e);
-                CH_HIP(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
-                (void)hipGraphDestroy(graph);
-            }
-            CH_HIP(hipGraphLaunch(ge, c->stream));
-        } else {
-            CH_TRY(enqueue_step(c, dt_mode, timed));
+        CH_TRY(launch_one_step(c, CHIMERA_DT_PRESCRIBED, true, false));
+        Fields6 src, dst;
+        for (int q = 0; q < 6; ++q) {
+            src.p[q] = c->S[c->cur][q];
+            dst.p[q] = traj[6 * (size_t)s + q];
         }
-        c->cur = 1 - c->cur;
+        hipLaunchKernelGGL(copy6_kernel, dim3(blocks), dim3(256), 0, c->stream, src, dst, n);
+        CH_HIP(hipGetLastError());
     }
     return CHIMERA_OK;
 }

@@ -39,6 +39,16 @@ struct GridView {
     double dx, vol, gamma;
 };
 
+// Stage intermediates the discrete adjoint reads back (owned cells only; written
+// only by the STORE instantiation of stage_tile).
+struct StageStore {
+    double *P[6];        // stage primitives rho, vx, vy, P, Bx, By
+    double *face[24];    // field f, side s (XL, XR, YL, YR) at 4*f + s, after the thermal check
+    unsigned long long *bits_x, *bits_y;  // MOOD 4f+s | thermal 24+s | face floor 32+4k+s (k: rho, P)
+    double *Up[4];       // after the flux update, before the floors
+    double *U3[4];       // after floors and the outflow correction, before the blend
+};
+
 template <int TX, int TY>
 struct TileShape {
     static constexpr int PW = TX + 4, PH = TY + 4;     // primitives with a 2-cell halo
@@ -77,7 +87,8 @@ template <int LIMITER>
 __device__ __forceinline__ unsigned face_pair(double m, double c, double p, double pp, double nA,
                                               double nB, bool last, bool use_lim, bool mood,
                                               bool rho_or_p, double inv_dx, double inv_2dx,
-                                              double hdx, double &fl, double &fr) {
+                                              double hdx, double &fl, double &fr,
+                                              unsigned &floor_bits) {
     double sl, sr;
     if (use_lim) {
         sl = limited_slope<LIMITER>(m, c, p, inv_dx);
@@ -96,7 +107,9 @@ __device__ __forceinline__ unsigned face_pair(double m, double c, double p, doub
         if (l < lo || l > hi) { l = c; bits |= 1u; }
         if (r < lo || r > hi) { r = c; bits |= 2u; }
     }
+    floor_bits = 0u;
     if (rho_or_p) {
+        floor_bits = (1.0e-8 > l ? 1u : 0u) | (1.0e-8 > r ? 2u : 0u);
         l = fmax(l, 1.0e-8);
         r = fmax(r, 1.0e-8);
     }
@@ -117,9 +130,10 @@ __device__ __forceinline__ bool thermal_reset(double f[6], const double c[6]) {
     return false;
 }
 
-template <int SOLVER, int LIMITER, int TX, int TY>
+template <int SOLVER, int LIMITER, int TX, int TY, bool STORE = false>
 __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool use_lim, bool mood,
-                                           int tile_x, int tile_y, double *lds) {
+                                           int tile_x, int tile_y, double *lds,
+                                           const StageStore *store = nullptr) {
     using S = TileShape<TX, TY>;
     constexpr int NT = TX * TY;
     const int tid = threadIdx.x;
@@ -152,6 +166,18 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         sP[3 * S::P_SZ + idx] = P;
         sP[4 * S::P_SZ + idx] = Bx;
         sP[5 * S::P_SZ + idx] = By;
+        if (STORE) {
+            int ci = lx - 2, cj = ly - 2;
+            if (ci >= 0 && ci < TX && cj >= 0 && cj < TY && x0 + ci < nx && y0 + cj < ny) {
+                size_t ko = (size_t)(x0 + ci) + (size_t)nx * (y0 + cj);
+                store->P[0][ko] = rho;
+                store->P[1][ko] = vx;
+                store->P[2][ko] = vy;
+                store->P[3][ko] = P;
+                store->P[4][ko] = Bx;
+                store->P[5][ko] = By;
+            }
+        }
     }
     __syncthreads();
 
@@ -167,19 +193,35 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             int ci = map_index(x0 + fx - 1, nx, px);
             bool last = !px && ci == nx - 1;
             double L[6], R[6], C[6];
-            unsigned bits = 0;
+            unsigned bits = 0, fbits = 0;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 const double *row = sP + q * S::P_SZ + ly * S::PW;
                 double c = row[lx];
                 C[q] = c;
+                unsigned fb;
                 bits |= face_pair<LIMITER>(row[lx - 1], c, row[lx + 1], row[lx + 2], row[lx + S::PW],
                                            row[lx - S::PW], last, use_lim, mood,
-                                           q == F_RHO || q == F_P, inv_dx, inv_2dx, hdx, L[q], R[q])
+                                           q == F_RHO || q == F_P, inv_dx, inv_2dx, hdx, L[q], R[q],
+                                           fb)
                         << (4 * q);
+                if (q == F_RHO) fbits |= fb;
+                if (q == F_P) fbits |= fb << 4;
             }
             if (thermal_reset(L, C)) bits |= 1u << 24;
             if (thermal_reset(R, C)) bits |= 1u << 25;
+            if (STORE) {
+                int i = fx - 1, j = fy - 1;
+                if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny) {
+                    size_t ko = (size_t)(x0 + i) + (size_t)nx * (y0 + j);
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        store->face[4 * q + 0][ko] = L[q];
+                        store->face[4 * q + 1][ko] = R[q];
+                    }
+                    store->bits_x[ko] = (unsigned long long)bits | ((unsigned long long)fbits << 32);
+                }
+            }
             FaceState a{L[F_RHO], L[F_VX], L[F_VY], L[F_P], L[F_BX], L[F_BY]};
             FaceState b{R[F_RHO], R[F_VX], R[F_VY], R[F_P], R[F_BX], R[F_BY]};
             Flux5 f = riemann<SOLVER>(a, b, gam);
@@ -200,19 +242,34 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             int cj = map_index(y0 + fy - 1, ny, py);
             bool last = !py && cj == ny - 1;
             double L[6], R[6], C[6];
-            unsigned bits = 0;
+            unsigned bits = 0, fbits = 0;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 const double *col = sP + q * S::P_SZ + ly * S::PW + lx;
                 double c = col[0];
                 C[q] = c;
+                unsigned fb;
                 bits |= face_pair<LIMITER>(col[-S::PW], c, col[S::PW], col[2 * S::PW], col[1], col[-1],
                                            last, use_lim, mood, q == F_RHO || q == F_P, inv_dx,
-                                           inv_2dx, hdx, L[q], R[q])
+                                           inv_2dx, hdx, L[q], R[q], fb)
                         << (4 * q + 2);
+                if (q == F_RHO) fbits |= fb << 2;
+                if (q == F_P) fbits |= fb << 6;
             }
             if (thermal_reset(L, C)) bits |= 1u << 26;
             if (thermal_reset(R, C)) bits |= 1u << 27;
+            if (STORE) {
+                int i = fx - 1, j = fy - 1;
+                if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny) {
+                    size_t ko = (size_t)(x0 + i) + (size_t)nx * (y0 + j);
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        store->face[4 * q + 2][ko] = L[q];
+                        store->face[4 * q + 3][ko] = R[q];
+                    }
+                    store->bits_y[ko] = (unsigned long long)bits | ((unsigned long long)fbits << 32);
+                }
+            }
             FaceState a{L[F_RHO], L[F_VY], L[F_VX], L[F_P], L[F_BY], L[F_BX]};
             FaceState b{R[F_RHO], R[F_VY], R[F_VX], R[F_P], R[F_BY], R[F_BX]};
             Flux5 f = riemann<SOLVER>(a, b, gam);
@@ -254,6 +311,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             if (ylo_open) v = v - dtdx * ft;
             else v = v - dtdx * ft + dtdx * sFY[q * S::FY_SZ + yr - S::FYW];
             U[q] = v;
+            if (STORE) store->Up[q][k] = v;
         }
         const int pc = (cj + 2) * S::PW + (ci + 2);
         const double Bxc = sP[4 * S::P_SZ + pc], Byc = sP[5 * S::P_SZ + pc];
@@ -298,6 +356,10 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             }
         }
 
+        if (STORE) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) store->U3[q][k] = U[q];
+        }
         if (stage > 1) {
             const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
             const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
