@@ -48,6 +48,13 @@ def parse_args():
     ap.add_argument("--grf-steps", type=int, default=500, help="steps of every GRF member")
     ap.add_argument("--plain-per-class", type=int, default=8,
                     help="members per grid size timed on the plain single-run path")
+    ap.add_argument("--adj-steps", type=int, default=1000,
+                    help="steps of the field-loop adjoint section (0: skip)")
+    ap.add_argument("--adj-fd-nodes", type=int, default=40,
+                    help="Az nodes checked against central differences in the adjoint section")
+    ap.add_argument("--dgrf-members", type=int, default=64,
+                    help="members of the differentiable GRF section (0: skip)")
+    ap.add_argument("--dgrf-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of 2^26 doubles (512 MiB each way) 3 "
                          "times, the reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -145,6 +152,198 @@ def batched_grf(args, torch) -> dict:
                          "avg_launch_ms": stage_ms, "launches": launches}}
 
 
+def adjoint_field_loop(args, torch) -> dict:
+    """Field loop 512 x 256 (periodic, HLLE, MC), dt frozen from a CFL run, loss =
+    sum((|B|^2_N - |B|^2_target)^2) with the target from Az (1 + 0.01 xi), xi seeded 7;
+    gradient with respect to every Az node through the checkpointed adjoint
+    (checkpoint every 50 steps).  cost ratio = backward wall time / wall time of the
+    same steps on the fused forward path.
+
+    Finite-difference check: the discrete map is not differentiable everywhere on this
+    state -- with MC a face value can land exactly on its MOOD bound (MC's 2*min branch),
+    where the map jumps, and outside the loop B is exactly 0 and rho, p, v exactly uniform,
+    where the limiter's dfL * dfR is exactly 0 (a kink) -- so central differences there do
+    not converge to the branch derivative.  The check therefore runs on the same grid with
+    the Van Leer limiter (faces strictly inside the MOOD bounds), Az plus 1e-4 x a smooth
+    periodic field and a smooth non-uniform background: central differences
+    (eps = 1e-6 ... 1e-9 max|Az|) of the forward path (bit-identical to the reference) at
+    seeded nodes inside the loop plus nodes near r = 0.3."""
+    import numpy as np
+    from chimera_amd import problems
+    from chimera_amd.adjoint import cell_b2, evolve, field_loop_state, record_dt
+    from chimera_amd.solver import Grid, DT_PRESCRIBED
+    nsteps, K = args.adj_steps, 50
+    rng = np.random.default_rng(7)
+
+    def setup(limiter):
+        prob, Az = problems.field_loop(512, 256, scheme=(2, limiter, 1, 1), return_az=True)
+        dts = record_dt(prob, nsteps)
+        return prob, Az, dts
+
+    def run_forward(g, dx, az_np, dts):
+        st = field_loop_state(torch.from_numpy(az_np).cuda(), dx)
+        g.upload(st)
+        g.steps(nsteps, dt_mode=DT_PRESCRIBED, dt=dts, graph=True)
+        out = torch.empty_like(st)
+        g.download(out)
+        return cell_b2(out)
+
+    def gradient(prob, Az, dts, T):
+        Azt = torch.from_numpy(Az).cuda().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        UN = evolve(field_loop_state(Azt, prob.dx), dts, prob, checkpoint_every=K)
+        loss = ((cell_b2(UN) - T) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return Azt.grad.cpu().numpy(), float(loss.detach()), t1 - t0, t2 - t1
+
+    # --- cost on the HLLE / MC configuration
+    prob, Az, dts = setup(2)
+    xi = rng.standard_normal(Az.shape)
+    Az_t = Az * (1.0 + 0.01 * xi)
+    g = Grid.from_problem(prob)
+    T = run_forward(g, prob.dx, Az_t, dts)
+    g.upload(field_loop_state(torch.from_numpy(Az).cuda(), prob.dx))
+    g.steps(2, dt_mode=DT_PRESCRIBED, dt=dts[:2], graph=True)
+    g.sync()
+    g.upload(field_loop_state(torch.from_numpy(Az).cuda(), prob.dx))
+    t0 = time.perf_counter()
+    g.steps(nsteps, dt_mode=DT_PRESCRIBED, dt=dts, graph=True)
+    g.sync()
+    t_fwd = time.perf_counter() - t0
+    g.close()
+    grad, loss_v, t_evolve, t_bwd = gradient(prob, Az, dts, T)
+    grad_finite = bool(np.isfinite(grad).all())
+
+    # --- finite differences on the differentiable (Van Leer, no uniform region) variant
+    fd = None
+    if args.adj_fd_nodes > 0:
+        ny, nx = Az.shape
+        hyd = problems.field_loop_background(nx, ny, seed=20)
+        Az_v = Az + 1.0e-4 * problems.smooth_periodic(nx, ny, seed=1)
+        prob_v = problems.field_loop_from_az(Az_v, prob.dx, scheme=(2, 1, 1, 1), hydro=hyd)
+        dts_v = record_dt(prob_v, nsteps)
+        hyd_t = [torch.from_numpy(a).cuda() for a in hyd]
+
+        def run_forward_bg(g, dx, az_np, dts):
+            st = field_loop_state(torch.from_numpy(az_np).cuda(), dx, hydro=hyd_t)
+            g.upload(st)
+            g.steps(nsteps, dt_mode=DT_PRESCRIBED, dt=dts, graph=True)
+            out = torch.empty_like(st)
+            g.download(out)
+            return cell_b2(out)
+
+        g = Grid.from_problem(prob_v)
+        Az_vt = Az_v * (1.0 + 0.01 * xi)
+        T_v = run_forward_bg(g, prob_v.dx, Az_vt, dts_v)
+
+        def loss_diff(ap, am):
+            # L(ap) - L(am) formed per cell before the sum (no cancellation of two
+            # nearly equal totals): sum (b+ - b-) (b+ + b- - 2 T)
+            bp = run_forward_bg(g, prob_v.dx, ap, dts_v)
+            bm = run_forward_bg(g, prob_v.dx, am, dts_v)
+            return float(((bp - bm) * (bp + bm - 2.0 * T_v)).sum())
+
+        Azt = torch.from_numpy(Az_v).cuda().requires_grad_(True)
+        UN = evolve(field_loop_state(Azt, prob_v.dx, hydro=hyd_t), dts_v, prob_v, checkpoint_every=K)
+        ((cell_b2(UN) - T_v) ** 2).sum().backward()
+        grad_v = Azt.grad.cpu().numpy()
+        x = -1.0 + (np.arange(nx) + 0.5) * prob.dx
+        y = -0.5 + (np.arange(ny) + 0.5) * prob.dx
+        X, Y = np.meshgrid(x, y, indexing="xy")
+        r = np.sqrt(X * X + Y * Y)
+        inside = np.flatnonzero(r.ravel() < 0.3)
+        ring = np.argsort(np.abs(r.ravel() - 0.3))[:64]
+        n_ring = min(8, args.adj_fd_nodes)
+        nodes = list(rng.choice(inside, size=args.adj_fd_nodes - n_ring, replace=False)) + \
+            list(rng.choice(ring, size=n_ring, replace=False))
+        gmax = float(np.abs(grad_v).max())
+        fd = {"variant": "Van Leer, Az + 1e-4 smooth field, smooth background (see docstring)",
+              "nodes": len(nodes),
+              "rel_err_definition": "|adjoint - FD| / max(|FD|, 1e-3 max|grad|)"}
+        for rel_eps in (1e-6, 1e-7, 1e-8, 1e-9):
+            eps = rel_eps * float(np.abs(Az_v).max())
+            rels = []
+            for idx in nodes:
+                j, i = divmod(int(idx), nx)
+                ap, am = Az_v.copy(), Az_v.copy()
+                ap[j, i] += eps
+                am[j, i] -= eps
+                d = loss_diff(ap, am) / (2.0 * eps)
+                rels.append(abs(grad_v[j, i] - d) / max(abs(d), 1e-3 * gmax))
+            rels = np.array(rels)
+            fd[f"eps_{rel_eps:g}_max|Az|"] = {"median_rel_err": float(np.median(rels)),
+                                              "max_rel_err": float(rels.max()),
+                                              "frac_le_1e-6": float(np.mean(rels <= 1e-6))}
+        g.close()
+    return {"workload": f"field loop 512x256 periodic, HLLE, MC, MUSCL+MOOD, CT, {nsteps} steps "
+                        f"(dt frozen from a CFL run), checkpoint every {K}",
+            "loss": loss_v, "grad_nodes": int(grad.size), "grad_finite": grad_finite,
+            "forward_s": t_fwd, "evolve_forward_s": t_evolve, "backward_s": t_bwd,
+            "cost_ratio": t_bwd / t_fwd, "fd_check": fd}
+
+
+def differentiable_grf(args, torch) -> dict:
+    """Differentiable GRF ensemble: members N in {384, 448, 512} (seed 20261203),
+    HLLD, MC, dt frozen from a CFL run; per-member loss = sum((rho_N - target)^2) with
+    a seeded target (5000 + m); gradient with respect to the white-noise fields that
+    generate psi and Az.  Members run one after another through the single-grid
+    adjoint.  Reports forward+backward cell-updates/s and backward / forward cost."""
+    import numpy as np
+    from chimera_amd import problems
+    from chimera_amd.adjoint import evolve, grf_member_from_noise, record_dt
+    from chimera_amd.solver import Grid, DT_PRESCRIBED
+    steps = args.dgrf_steps
+    sizes = problems.grf_sizes(args.dgrf_members, choices=(384, 448, 512))
+    cells = sum(n * n for n in sizes)
+    t_fwd = t_total = t_bwd = 0.0
+    gnorm = []
+    for m, n in enumerate(sizes):
+        gen = torch.Generator(device="cuda").manual_seed(1000 + m)
+        w = torch.randn((2, n, n), generator=gen, device="cuda", dtype=torch.float64)
+        params = problems.grf_params(1000 + m)
+        prob, st0 = grf_member_from_noise(w, n, params, scheme=(3, 2, 1, 1))
+        st0 = st0.detach()
+        dts = record_dt(prob, steps)
+        tg = torch.Generator(device="cuda").manual_seed(5000 + m)
+        target = 1.0 + 0.05 * torch.randn((n, n), generator=tg, device="cuda", dtype=torch.float64)
+        with Grid.from_problem(prob) as g:
+            g.steps(1, dt_mode=DT_PRESCRIBED, dt=dts[:1], graph=True)
+            g.upload(st0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g.steps(steps, dt_mode=DT_PRESCRIBED, dt=dts, graph=True)
+            g.sync()
+            t_fwd += time.perf_counter() - t0
+        wv = w.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, U0 = grf_member_from_noise(wv, n, params, scheme=(3, 2, 1, 1))
+        UN = evolve(U0, dts, prob, checkpoint_every=50)
+        rho = UN[0] / (prob.dx * prob.dx)
+        loss = ((rho - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_total += t2 - t0
+        t_bwd += t2 - t1
+        gnorm.append(float(wv.grad.norm()))
+    return {"workload": f"differentiable GRF ensemble: {len(sizes)} members, N in {{384,448,512}}, "
+                        f"HLLD, MC, MUSCL+MOOD, CT, {steps} steps each (dt frozen), gradient wrt "
+                        "the psi / Az noise fields; members run sequentially",
+            "cells": cells, "steps": steps,
+            "forward_value": cells * steps / t_fwd,
+            "fwd_bwd_value": cells * steps / t_total, "unit": "cell-updates/s",
+            "cost_ratio": t_bwd / t_fwd if t_fwd > 0 else None,
+            "all_grads_finite": bool(np.all(np.isfinite(gnorm)))}
+
+
 def traffic_from_profiles(n: int) -> float | None:
     """HBM bytes per stage-kernel launch from the committed rocprofv3 PMC summary."""
     p = ROOT / "profiles" / f"stage_kernel_rotor{n}_pmc.md"
@@ -209,6 +408,8 @@ def main():
     g.close()
 
     batched = batched_grf(args, torch) if args.members > 0 else None
+    adjoint = adjoint_field_loop(args, torch) if args.adj_steps > 0 else None
+    dgrf = differentiable_grf(args, torch) if args.dgrf_members > 0 else None
 
     value = cells * args.steps / elapsed
     achieved = BYTES_PER_CELL_STAGE * cells / (stage_ms * 1e-3) / 1e9 if stage_ms > 0 else None
@@ -238,6 +439,8 @@ def main():
         "graph_value": graph_value,
         "finite": finite,
         "batched_grf": batched,
+        "adjoint_field_loop": adjoint,
+        "differentiable_grf": dgrf,
     }
     print(json.dumps(out))
 

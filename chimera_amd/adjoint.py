@@ -182,3 +182,13 @@ def field_loop_state(Az: torch.Tensor, dx: float, gamma: float = 5.0 / 3.0,
 def cell_b2(U: torch.Tensor) -> torch.Tensor:
     Bx, By = cell_b_periodic(U[4], U[5])
     return Bx * Bx + By * By
+
+
+def grf_member_from_noise(w: torch.Tensor, n: int, params: dict, scheme=(3, 2, 1, 1)):
+    """GRF member from its white-noise fields w (2, n, n) (differentiable in w):
+    returns (Problem with a detached copy of the state, state tensor)."""
+    from . import problems
+    st = problems.grf_state_from_noise(w, n, params)
+    prob = problems.Problem(f"grf_{n}", st.detach(), 1.0 / n, params["gamma"], problems.PERIODIC,
+                            tuple(scheme))
+    return prob, st

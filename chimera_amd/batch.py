@@ -51,8 +51,11 @@ class Ensemble:
             assert st.shape == (6,) + self.shapes[member]
             ptrs = (C.c_void_p * 6)(*[st[q].ctypes.data for q in range(6)])
         else:
+            import torch
             st = state.contiguous()
             assert tuple(st.shape) == (6,) + self.shapes[member] and str(st.dtype) == "torch.float64"
+            # the copy runs on the library's stream: finish torch's work on the tensor first
+            torch.cuda.synchronize()
             ptrs = (C.c_void_p * 6)(*[st[q].data_ptr() for q in range(6)])
         check(self._L.chimera_batch_upload(self._h, int(member), ptrs), "chimera_batch_upload")
 

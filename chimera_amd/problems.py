@@ -284,30 +284,33 @@ def grf_sizes(members: int, choices=(128, 256, 384, 512), seed: int = 20261203) 
     return [int(c) for c in rng.choice(choices, size=members)]
 
 
-def grf_member_device(n: int, seed: int, scheme=(1, 2, 1, 1), device="cuda", **kw):
-    """The ensemble member of ``grf_member`` generated on the GPU (torch FFTs): the
-    physical parameters come from the same numpy draw, the two white-noise fields from
-    a torch generator seeded with ``seed``.  The returned Problem's ``state`` is a
-    float64 tensor (6, n, n) on ``device``."""
-    import torch
+def grf_params(seed: int, gamma_range=(1.4, 5.0 / 3.0), v0_range=(0.1, 1.0),
+               beta_range=(0.1, 10.0), ell_range=(0.05, 0.15), cf_cap: float = 3.0) -> dict:
+    """The physical parameters of GRF member ``seed`` (the numpy draw grf_member makes
+    first): gamma, rms |v|, plasma beta, RBF length and the signal-speed cap."""
     rng = np.random.default_rng(seed)
-    gamma = rng.uniform(*kw.get("gamma_range", (1.4, 5.0 / 3.0)))
-    v0 = rng.uniform(*kw.get("v0_range", (0.1, 1.0)))
-    beta = rng.uniform(*kw.get("beta_range", (0.1, 10.0)))
-    ell = rng.uniform(*kw.get("ell_range", (0.05, 0.15)))
-    cf_cap = kw.get("cf_cap", 3.0)
+    return {"gamma": rng.uniform(*gamma_range), "v0": rng.uniform(*v0_range),
+            "beta": rng.uniform(*beta_range), "ell": rng.uniform(*ell_range), "cf_cap": cf_cap}
+
+
+def grf_state_from_noise(w, n: int, params: dict):
+    """GRF member state (6, n, n) from two white-noise fields w = (w_psi, w_Az) in torch,
+    differentiable in w: RBF smoothing by FFT, v = centred curl of psi scaled to rms v0,
+    face B = backward curl of Az scaled to B_rms = sqrt(2 p / beta), v scaled so that
+    max(c_f + |v|) <= cf_cap, rho = p = 1 (as grf_member)."""
+    import torch
+    dev = w.device
+    gamma, v0, beta, ell, cf_cap = (params[k] for k in ("gamma", "v0", "beta", "ell", "cf_cap"))
     dx = 1.0 / n
-    gen = torch.Generator(device=device).manual_seed(int(seed))
-    k = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float64) * TWO_PI
+    k = torch.fft.fftfreq(n, d=1.0 / n, device=dev, dtype=torch.float64) * TWO_PI
     S = torch.exp(-0.5 * ell * ell * (k[None, :] ** 2 + k[:, None] ** 2))
 
-    def field():
-        w = torch.randn((n, n), generator=gen, device=device, dtype=torch.float64)
-        f = torch.fft.ifft2(torch.fft.fft2(w) * S).real
+    def smooth(x):
+        f = torch.fft.ifft2(torch.fft.fft2(x) * S).real
         f = f - f.mean()
         return f / torch.sqrt(torch.mean(f * f))
 
-    psi, Az = field(), field()
+    psi, Az = smooth(w[0]), smooth(w[1])
     vx = (torch.roll(psi, -1, 0) - torch.roll(psi, 1, 0)) / (2.0 * dx)
     vy = -(torch.roll(psi, -1, 1) - torch.roll(psi, 1, 1)) / (2.0 * dx)
     v_rms = torch.sqrt(torch.mean(vx * vx + vy * vy))
@@ -327,6 +330,19 @@ def grf_member_device(n: int, seed: int, scheme=(1, 2, 1, 1), device="cuda", **k
     half_b2 = 0.5 * (Bx * Bx + By * By)
     M = torch.full_like(vx, vol)
     E = (1.0 / (gamma - 1.0) + 0.5 * (vx * vx + vy * vy) + half_b2) * vol
-    st = torch.stack([M, M * vx, M * vy, E, bx, by]).contiguous()
-    return Problem(f"grf_{n}_{seed}", st, dx, gamma, PERIODIC, tuple(scheme),
-                   {"seed": seed, "ell": ell, "v0": v0, "beta": beta})
+    return torch.stack([M, M * vx, M * vy, E, bx, by]).contiguous()
+
+
+def grf_member_device(n: int, seed: int, scheme=(1, 2, 1, 1), device="cuda", **kw):
+    """The ensemble member of ``grf_member`` generated on the GPU (torch FFTs): the
+    physical parameters come from the same numpy draw, the two white-noise fields from
+    a torch generator seeded with ``seed``.  The returned Problem's ``state`` is a
+    float64 tensor (6, n, n) on ``device``."""
+    import torch
+    params = grf_params(seed, **kw)
+    gen = torch.Generator(device=device).manual_seed(int(seed))
+    w = torch.stack([torch.randn((n, n), generator=gen, device=device, dtype=torch.float64)
+                     for _ in range(2)])
+    st = grf_state_from_noise(w, n, params)
+    return Problem(f"grf_{n}_{seed}", st, 1.0 / n, params["gamma"], PERIODIC, tuple(scheme),
+                   {"seed": seed, "ell": params["ell"], "v0": params["v0"], "beta": params["beta"]})

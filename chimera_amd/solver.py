@@ -50,8 +50,11 @@ class Grid:
             assert st.shape == (6, self.ny, self.nx), st.shape
             check(self._L.chimera_ctx_upload(self._h, _host_ptrs(st)), "chimera_ctx_upload")
         else:
+            import torch
             t = state.contiguous()
             assert tuple(t.shape) == (6, self.ny, self.nx) and str(t.dtype) == "torch.float64"
+            # the copy runs on the library's stream: finish torch's work on the tensor first
+            torch.cuda.synchronize()
             ptrs = (C.c_void_p * 6)(*[t[q].data_ptr() for q in range(6)])
             check(self._L.chimera_ctx_upload(self._h, ptrs), "chimera_ctx_upload")
 
@@ -61,6 +64,8 @@ class Grid:
         if isinstance(out, np.ndarray):
             check(self._L.chimera_ctx_download(self._h, _host_ptrs(out)), "chimera_ctx_download")
         else:
+            import torch
+            torch.cuda.synchronize()   # torch may still be using the destination
             ptrs = (C.c_void_p * 6)(*[out[q].data_ptr() for q in range(6)])
             check(self._L.chimera_ctx_download(self._h, ptrs), "chimera_ctx_download")
         return out

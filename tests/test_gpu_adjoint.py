@@ -205,3 +205,29 @@ def test_state_gradient_vs_reference_fd_mc(gpu_lib, solver):
             fd = (loss_ref(sp) - loss_ref(sm)) / (2 * eps)
             worst = max(worst, abs(fd - grad[q, j, i]) / abs(fd))
     assert worst <= 1e-6, worst
+
+
+@pytest.mark.parametrize("scheme", [(2, 2, 1, 1), (2, 1, 1, 1)])
+def test_evolve_gradient_exact_midsize(gpu_lib, scheme):
+    """128 x 64 cone field loop, 60 steps in segments of 25 (many tiles, several
+    segments): the GPU gradient against autograd through the restatement."""
+    torch = _torch()
+    from chimera_amd.adjoint import cell_b2, evolve, field_loop_state, record_dt
+    from tests.adjoint_reference import Geometry, step
+    nsteps = 60
+    p, Az = problems.field_loop(128, 64, scheme=scheme, return_az=True)
+    dts = record_dt(p, nsteps)
+    T = np.random.default_rng(9).uniform(0.5, 1.5, Az.shape) * 1e-6
+    Azg = torch.from_numpy(Az).cuda().requires_grad_(True)
+    UN = evolve(field_loop_state(Azg, p.dx), dts, p, checkpoint_every=25)
+    ((cell_b2(UN) - torch.from_numpy(T).cuda()) ** 2).sum().backward()
+    got = Azg.grad.cpu().numpy()
+    Azc = torch.from_numpy(Az.copy()).requires_grad_(True)
+    U = field_loop_state(Azc, p.dx)
+    G = Geometry.of(p)
+    for k in range(nsteps):
+        U = step(U, float(dts[k]), G)
+    ((cell_b2(U) - torch.from_numpy(T)) ** 2).sum().backward()
+    ref = Azc.grad.numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= 1e-11, err
