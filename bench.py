@@ -382,6 +382,8 @@ def main():
     g = Grid.from_problem(prob)
     g.steps(args.warmup, dt_mode=DT_CFL)
     g.sync()
+    tile_shape, tile_ms = g.autotune(reps=3)   # times one stage per shape; state unchanged
+    tile = g.tile_shape()
 
     # timed region: eager launches with HIP events around every fused stage kernel
     g.set_timing(True)
@@ -428,7 +430,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"MHD rotor {n}x{n}, outflow, gamma=1.4, HLLE, Van Leer, "
                                "MUSCL+MOOD, CT, SSP-RK3, CFL 0.3 (pure CFL dt)",
-                   "nx": n, "ny": n, "path": "plain resident (fused stage kernel)"},
+                   "nx": n, "ny": n, "path": "plain resident (fused stage kernel)",
+                   "tile": [tile[1], tile[2]],
+                   "tile_autotune_ms": {f"{a}x{b}": round(t, 4) for (a, b), t in
+                                        zip(((32, 8), (16, 16), (64, 4), (32, 32)), tile_ms)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic_from_profiles(n),

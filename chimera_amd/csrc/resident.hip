@@ -47,9 +47,8 @@ int check_scheme(const chimera_scheme_t *s) {
 // kernels
 
 template <int SOLVER, int LIMITER, int TX, int TY>
-__global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(GridView g, int stage, int use_lim,
-                                                             int mood, int tiles_x, int ntiles) {
-    static_assert(TX * TY == STAGE_THREADS, "one cell per thread");
+__global__ __launch_bounds__(TX * TY) void stage_kernel(GridView g, int stage, int use_lim,
+                                                       int mood, int tiles_x, int ntiles) {
     __shared__ double lds[TileShape<TX, TY>::LDS_DOUBLES];
     const int b = blockIdx.x, nb = gridDim.x;  // nb is a multiple of 8
     const int tile = (b % 8) * (nb / 8) + b / 8;
@@ -61,16 +60,21 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(GridView g, int st
 using StageFn = void (*)(GridView, int, int, int, int, int);
 
 #define CH_SHAPES(S, L)                                                                 \
-    {stage_kernel<S, L, 32, 8>, stage_kernel<S, L, 16, 16>, stage_kernel<S, L, 64, 4>}
+    {stage_kernel<S, L, 32, 8>, stage_kernel<S, L, 16, 16>, stage_kernel<S, L, 64, 4>,     \
+     stage_kernel<S, L, 32, 32>}
+#define CH_SHAPES_256(S, L)                                                             \
+    {stage_kernel<S, L, 32, 8>, stage_kernel<S, L, 16, 16>, stage_kernel<S, L, 64, 4>, nullptr}
 
+// nullptr: shape not available for that solver
 StageFn stage_fn(int solver, int limiter, int shape) {
-    static const StageFn table[3][2][NUM_SHAPES] = {
+    static const StageFn table[3][2][RES_NUM_SHAPES] = {
         {CH_SHAPES(RUSANOV, LIM_VAN_LEER), CH_SHAPES(RUSANOV, LIM_MC)},
         {CH_SHAPES(HLLE, LIM_VAN_LEER), CH_SHAPES(HLLE, LIM_MC)},
-        {CH_SHAPES(HLLD, LIM_VAN_LEER), CH_SHAPES(HLLD, LIM_MC)}};
+        {CH_SHAPES_256(HLLD, LIM_VAN_LEER), CH_SHAPES_256(HLLD, LIM_MC)}};
     return table[solver - 1][limiter - 1][shape];
 }
 #undef CH_SHAPES
+#undef CH_SHAPES_256
 
 __device__ __forceinline__ double wave_max(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -186,7 +190,9 @@ __global__ __launch_bounds__(256) void divb_reduce_kernel(const double *bx, cons
 
 int launch_stage(const StageLaunch &L, hipStream_t stream) {
     StageFn fn = stage_fn(L.solver, L.limiter, L.shape);
-    hipLaunchKernelGGL(fn, dim3(L.grid), dim3(STAGE_THREADS), 0, stream, L.g, L.stage, L.use_lim,
+    CH_REQUIRE(fn, "tile shape %d is not available for riemann_solver %d", L.shape, L.solver);
+    const TileDims d = RES_TILE_SHAPES[L.shape];
+    hipLaunchKernelGGL(fn, dim3(L.grid), dim3(d.tx * d.ty), 0, stream, L.g, L.stage, L.use_lim,
                        L.mood, L.tiles_x, L.ntiles);
     return launch_check("stage_kernel");
 }
@@ -301,7 +307,7 @@ static void drop_graphs(chimera_ctx *c) {
 
 static void set_shape(chimera_ctx *c, int shape) {
     c->shape = shape;
-    const TileDims d = TILE_SHAPES[shape];
+    const TileDims d = RES_TILE_SHAPES[shape];
     c->tiles_x = (c->nx + d.tx - 1) / d.tx;
     c->tiles_y = (c->ny + d.ty - 1) / d.ty;
     c->ntiles = c->tiles_x * c->tiles_y;
@@ -672,7 +678,11 @@ extern "C" int chimera_ctx_stage_kernel_ms(chimera_ctx_t *c, double *ms, int *la
 
 extern "C" int chimera_ctx_set_tile_shape(chimera_ctx_t *c, int shape) {
     CH_REQUIRE(c, "ctx is NULL");
-    CH_REQUIRE(shape >= 0 && shape < NUM_SHAPES, "tile shape must be 0..%d", NUM_SHAPES - 1);
+    CH_REQUIRE(shape >= 0 && shape < RES_NUM_SHAPES, "tile shape must be 0..%d",
+               RES_NUM_SHAPES - 1);
+    CH_REQUIRE(stage_fn(c->sch.riemann_solver, c->sch.slope_limiter, shape),
+               "tile shape %d is not available for riemann_solver %d", shape,
+               c->sch.riemann_solver);
     CH_HIP(hipStreamSynchronize(c->stream));
     set_shape(c, shape);
     return CHIMERA_OK;
@@ -681,14 +691,14 @@ extern "C" int chimera_ctx_set_tile_shape(chimera_ctx_t *c, int shape) {
 extern "C" int chimera_ctx_get_tile_shape(chimera_ctx_t *c, int *shape, int *tx, int *ty) {
     CH_REQUIRE(c, "ctx is NULL");
     if (shape) *shape = c->shape;
-    if (tx) *tx = TILE_SHAPES[c->shape].tx;
-    if (ty) *ty = TILE_SHAPES[c->shape].ty;
+    if (tx) *tx = RES_TILE_SHAPES[c->shape].tx;
+    if (ty) *ty = RES_TILE_SHAPES[c->shape].ty;
     return CHIMERA_OK;
 }
 
 // Time each tile shape on one stage of the current state (output to the scratch
 // copy, the state is not advanced) and keep the fastest.  ms_out (optional) gets
-// NUM_SHAPES timings.
+// CHIMERA_CTX_NUM_SHAPES timings (-1 for a shape the solver does not have).
 extern "C" int chimera_ctx_autotune(chimera_ctx_t *c, int reps, int *best, double *ms_out) {
     CH_REQUIRE(c, "ctx is NULL");
     if (reps < 1) reps = 3;
@@ -697,7 +707,11 @@ extern "C" int chimera_ctx_autotune(chimera_ctx_t *c, int reps, int *best, doubl
     CH_HIP(hipEventCreate(&b));
     double best_ms = 1e300;
     int best_shape = c->shape;
-    for (int sh = 0; sh < NUM_SHAPES; ++sh) {
+    for (int sh = 0; sh < RES_NUM_SHAPES; ++sh) {
+        if (!stage_fn(c->sch.riemann_solver, c->sch.slope_limiter, sh)) {
+            if (ms_out) ms_out[sh] = -1.0;
+            continue;
+        }
         set_shape(c, sh);
         StageLaunch L = stage_launch(c, 2, c->S[c->cur], c->S[c->cur], c->S[2], nullptr);
         CH_TRY(launch_stage(L, c->stream));  // warm-up

@@ -19,6 +19,12 @@ struct TileDims {
 constexpr int NUM_SHAPES = 3;
 constexpr TileDims TILE_SHAPES[NUM_SHAPES] = {{32, 8}, {16, 16}, {64, 4}};
 constexpr int STAGE_THREADS = 256;
+// The single-grid driver also has a 32 x 32 tile run by 1024 threads (Rusanov and
+// HLLE; HLLD needs more than the 128 VGPRs a 1024-thread block leaves each lane): per
+// cell 1.27 primitive conversions and 2.19 Riemann faces instead of 1.69 and 2.48 for
+// 32 x 8, in 148 KiB of LDS, one block (16 waves) per CU.
+constexpr int RES_NUM_SHAPES = 4;
+constexpr TileDims RES_TILE_SHAPES[RES_NUM_SHAPES] = {{32, 8}, {16, 16}, {64, 4}, {32, 32}};
 
 // Per-grid step bookkeeping kept on the device so whole steps can be captured.
 struct StepState {

@@ -78,25 +78,22 @@ __device__ __forceinline__ int map_index(int g, int n, int periodic) {
 // Field order in LDS / face registers: rho, vx, vy, P, Bx, By.
 enum { F_RHO = 0, F_VX = 1, F_VY = 2, F_P = 3, F_BX = 4, F_BY = 5 };
 
-// Face states of one field at the two faces a thread handles: the MUSCL left state
-// (cell c extrapolated forward), the right state (next cell extrapolated backward,
-// or the cell value itself at a non-periodic hi boundary), then MOOD against the
-// 5-point stencil of cell c and the rho/P floor.  Returns the MOOD bits (1: left
-// reset, 2: right reset).
+// Limited (or central) slope of a cell from its two neighbours along one direction.
 template <int LIMITER>
-__device__ __forceinline__ unsigned face_pair(double m, double c, double p, double pp, double nA,
-                                              double nB, bool last, bool use_lim, bool mood,
-                                              bool rho_or_p, double inv_dx, double inv_2dx,
-                                              double hdx, double &fl, double &fr,
-                                              unsigned &floor_bits) {
-    double sl, sr;
-    if (use_lim) {
-        sl = limited_slope<LIMITER>(m, c, p, inv_dx);
-        sr = limited_slope<LIMITER>(c, p, pp, inv_dx);
-    } else {
-        sl = central_slope(m, p, inv_2dx);
-        sr = central_slope(c, pp, inv_2dx);
-    }
+__device__ __forceinline__ double cell_slope(double m, double c, double p, bool use_lim,
+                                             double inv_dx, double inv_2dx) {
+    return use_lim ? limited_slope<LIMITER>(m, c, p, inv_dx) : central_slope(m, p, inv_2dx);
+}
+
+// Face states of one field at one face from the slopes of its two cells: the MUSCL
+// left state (cell c extrapolated forward), the right state (next cell extrapolated
+// backward, or the cell value itself at a non-periodic hi boundary), then MOOD against
+// the 5-point stencil of cell c (neighbours m, p, nA, nB) and the rho/P floor.  Returns
+// the MOOD bits (1: left reset, 2: right reset); floor_bits likewise for the floor.
+__device__ __forceinline__ unsigned face_states(double m, double c, double p, double nA,
+                                                double nB, double sl, double sr, bool last,
+                                                bool mood, bool rho_or_p, double hdx, double &fl,
+                                                double &fr, unsigned &floor_bits) {
     double l = c + hdx * sl;
     double r = last ? c : p - hdx * sr;
     unsigned bits = 0;
@@ -117,6 +114,71 @@ __device__ __forceinline__ unsigned face_pair(double m, double c, double p, doub
     fr = r;
     return bits;
 }
+
+// Work split of phase 2: every face line (x faces along a row, y faces along a column)
+// is cut into runs of consecutive faces, as many runs as threads (runs as even as
+// possible, the longest lines cut most), ordered longest first so that runs of one
+// length fill whole waves.  Built at compile time per tile shape.
+struct FaceRun {
+    int dir, line, start, len;
+};
+
+template <int TX, int TY>
+struct FaceRunTable {
+    static constexpr int NT = TX * TY;
+    static constexpr int NXL = TY + 2, XLEN = TX + 1;   // x-face rows
+    static constexpr int NYL = TX + 2, YLEN = TY + 1;   // y-face columns
+    static constexpr int NL = NXL + NYL;
+    FaceRun r[NT];
+    constexpr FaceRunTable() : r{} {
+        int cnt[NL] = {};
+        int lmax = 1;
+        // the shortest maximum run length whose runs fit in NT threads
+        for (;; ++lmax) {
+            int total = 0;
+            for (int l = 0; l < NL; ++l) total += (len_of(l) + lmax - 1) / lmax;
+            if (total <= NT) break;
+        }
+        int total = 0;
+        for (int l = 0; l < NL; ++l) {
+            cnt[l] = (len_of(l) + lmax - 1) / lmax;
+            total += cnt[l];
+        }
+        while (total < NT) {   // one more run on the line with the longest runs
+            int best = -1, longest = 1;
+            for (int l = 0; l < NL; ++l) {
+                int m = len_of(l);
+                if (cnt[l] >= m) continue;
+                int lr = (m + cnt[l] - 1) / cnt[l];
+                if (lr > longest) {
+                    longest = lr;
+                    best = l;
+                }
+            }
+            if (best < 0) break;
+            ++cnt[best];
+            ++total;
+        }
+        int t = 0;
+        for (int want = lmax; want >= 1; --want)
+            for (int l = 0; l < NL; ++l) {
+                int m = len_of(l), c = cnt[l], base = m / c, extra = m % c, pos = 0;
+                for (int k = 0; k < c; ++k) {
+                    int len = base + (k < extra ? 1 : 0);
+                    if (len == want && t < NT) {
+                        r[t] = FaceRun{l < NXL ? 0 : 1, l < NXL ? l : l - NXL, pos, len};
+                        ++t;
+                    }
+                    pos += len;
+                }
+            }
+        for (; t < NT; ++t) r[t] = FaceRun{0, 0, 0, 0};
+    }
+    static constexpr int len_of(int l) { return l < NXL ? XLEN : YLEN; }
+};
+
+template <int TX, int TY>
+__device__ constexpr FaceRunTable<TX, TY> kFaceRuns{};
 
 // Thermal-pressure check of one face (src/mhd_derivatives.f90:262-267): when the
 // implied thermal pressure is below P_floor every field takes the cell value.
@@ -183,106 +245,83 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
 
     const bool want_mask = g.mask != nullptr;
 
-    // ---- phase 2: x faces, then y faces (rotated: normal vy / By), in one loop so
-    //      that no wave idles between the two face sets ----------------------------
-    for (int face = tid; face < S::FX_SZ + S::FY_SZ; face += NT) {
-        if (face < S::FX_SZ) {
-            const int idx = face;
-            int fx = idx % S::FXW, fy = idx / S::FXW;  // face i = fx-1, row j = fy-1
-            int lx = fx + 1, ly = fy + 1;                  // LDS position of the left cell
-            int ci = map_index(x0 + fx - 1, nx, px);
-            bool last = !px && ci == nx - 1;
+    // ---- phase 2: every x face and y face the tile needs, as runs of consecutive faces
+    //      along their normal (one run per thread): the slope of a run's right cell is
+    //      the next face's left-cell slope, so each slope is evaluated once per run.  x
+    //      and y runs share one code path (strides and field roles by direction), so a
+    //      wave never splits on the direction. ------------------------------------------
+    {
+        const FaceRun run = kFaceRuns<TX, TY>.r[tid];
+        const int dir = run.dir;                        // 0: x faces, 1: y faces (rotated)
+        const int sA = dir ? S::PW : 1;                 // LDS stride along the face normal
+        const int sT = dir ? 1 : S::PW;                 // LDS stride across it
+        // slope of the first face's left cell; afterwards each face's right-cell slope
+        // is carried to the next face
+        double sprev[6];
+        if (run.len > 0) {
+            const int fx0 = dir ? run.line : run.start, fy0 = dir ? run.start : run.line;
+            const int lpos0 = (fy0 + 1) * S::PW + (fx0 + 1);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const double *b = sP + q * S::P_SZ + lpos0;
+                sprev[q] = cell_slope<LIMITER>(b[-sA], b[0], b[sA], use_lim, inv_dx, inv_2dx);
+            }
+        }
+        for (int t = 0; t < run.len; ++t) {
+            // x face: i = fx-1 in row j = fy-1, left cell at LDS (fx+1, fy+1);
+            // y face: column i = fx-1, j = fy-1, lower cell at LDS (fx+1, fy+1)
+            const int fx = dir ? run.line : run.start + t;
+            const int fy = dir ? run.start + t : run.line;
+            const int lpos = (fy + 1) * S::PW + (fx + 1);
+            const bool last = dir ? (!py && map_index(y0 + fy - 1, ny, py) == ny - 1)
+                                  : (!px && map_index(x0 + fx - 1, nx, px) == nx - 1);
             double L[6], R[6], C[6];
             unsigned bits = 0, fbits = 0;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double *row = sP + q * S::P_SZ + ly * S::PW;
-                double c = row[lx];
+                const double *b = sP + q * S::P_SZ + lpos;
+                const double m = b[-sA], c = b[0], p = b[sA], pp = b[2 * sA];
                 C[q] = c;
+                const double sl = sprev[q];
+                const double sr = cell_slope<LIMITER>(c, p, pp, use_lim, inv_dx, inv_2dx);
+                sprev[q] = sr;
                 unsigned fb;
-                bits |= face_pair<LIMITER>(row[lx - 1], c, row[lx + 1], row[lx + 2], row[lx + S::PW],
-                                           row[lx - S::PW], last, use_lim, mood,
-                                           q == F_RHO || q == F_P, inv_dx, inv_2dx, hdx, L[q], R[q],
-                                           fb)
-                        << (4 * q);
-                if (q == F_RHO) fbits |= fb;
-                if (q == F_P) fbits |= fb << 4;
+                bits |= face_states(m, c, p, b[sT], b[-sT], sl, sr, last, mood,
+                                    q == F_RHO || q == F_P, hdx, L[q], R[q], fb)
+                        << (4 * q + 2 * dir);
+                if (q == F_RHO) fbits |= fb << (2 * dir);
+                if (q == F_P) fbits |= fb << (4 + 2 * dir);
             }
-            if (thermal_reset(L, C)) bits |= 1u << 24;
-            if (thermal_reset(R, C)) bits |= 1u << 25;
-            if (STORE) {
-                int i = fx - 1, j = fy - 1;
-                if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny) {
-                    size_t ko = (size_t)(x0 + i) + (size_t)nx * (y0 + j);
+            if (thermal_reset(L, C)) bits |= 1u << (24 + 2 * dir);
+            if (thermal_reset(R, C)) bits |= 1u << (25 + 2 * dir);
+            const int i = fx - 1, j = fy - 1;
+            const bool owned = i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny;
+            if (STORE && owned) {
+                size_t ko = (size_t)(x0 + i) + (size_t)nx * (y0 + j);
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) {
-                        store->face[4 * q + 0][ko] = L[q];
-                        store->face[4 * q + 1][ko] = R[q];
-                    }
-                    store->bits_x[ko] = (unsigned long long)bits | ((unsigned long long)fbits << 32);
+                for (int q = 0; q < 6; ++q) {
+                    store->face[4 * q + 2 * dir][ko] = L[q];
+                    store->face[4 * q + 2 * dir + 1][ko] = R[q];
                 }
+                unsigned long long w = (unsigned long long)bits | ((unsigned long long)fbits << 32);
+                if (dir) store->bits_y[ko] = w;
+                else store->bits_x[ko] = w;
             }
-            FaceState a{L[F_RHO], L[F_VX], L[F_VY], L[F_P], L[F_BX], L[F_BY]};
-            FaceState b{R[F_RHO], R[F_VX], R[F_VY], R[F_P], R[F_BX], R[F_BY]};
-            Flux5 f = riemann<SOLVER>(a, b, gam);
-            sFX[0 * S::FX_SZ + idx] = f.mass;
-            sFX[1 * S::FX_SZ + idx] = f.mom_n;
-            sFX[2 * S::FX_SZ + idx] = f.mom_t;
-            sFX[3 * S::FX_SZ + idx] = f.energy;
-            sFX[4 * S::FX_SZ + idx] = f.bt;
-            if (want_mask) {
-                int i = fx - 1, j = fy - 1;
-                if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny && bits)
-                    atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
-            }
-        } else {
-            const int idx = face - S::FX_SZ;
-            int fx = idx % S::FYW, fy = idx / S::FYW;      // face column i = fx-1, j = fy-1
-            int lx = fx + 1, ly = fy + 1;                  // LDS position of the lower cell
-            int cj = map_index(y0 + fy - 1, ny, py);
-            bool last = !py && cj == ny - 1;
-            double L[6], R[6], C[6];
-            unsigned bits = 0, fbits = 0;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const double *col = sP + q * S::P_SZ + ly * S::PW + lx;
-                double c = col[0];
-                C[q] = c;
-                unsigned fb;
-                bits |= face_pair<LIMITER>(col[-S::PW], c, col[S::PW], col[2 * S::PW], col[1], col[-1],
-                                           last, use_lim, mood, q == F_RHO || q == F_P, inv_dx,
-                                           inv_2dx, hdx, L[q], R[q], fb)
-                        << (4 * q + 2);
-                if (q == F_RHO) fbits |= fb << 2;
-                if (q == F_P) fbits |= fb << 6;
-            }
-            if (thermal_reset(L, C)) bits |= 1u << 26;
-            if (thermal_reset(R, C)) bits |= 1u << 27;
-            if (STORE) {
-                int i = fx - 1, j = fy - 1;
-                if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny) {
-                    size_t ko = (size_t)(x0 + i) + (size_t)nx * (y0 + j);
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) {
-                        store->face[4 * q + 2][ko] = L[q];
-                        store->face[4 * q + 3][ko] = R[q];
-                    }
-                    store->bits_y[ko] = (unsigned long long)bits | ((unsigned long long)fbits << 32);
-                }
-            }
-            FaceState a{L[F_RHO], L[F_VY], L[F_VX], L[F_P], L[F_BY], L[F_BX]};
-            FaceState b{R[F_RHO], R[F_VY], R[F_VX], R[F_P], R[F_BY], R[F_BX]};
-            Flux5 f = riemann<SOLVER>(a, b, gam);
-            sFY[0 * S::FY_SZ + idx] = f.mass;
-            sFY[1 * S::FY_SZ + idx] = f.mom_t;   // Momx through y faces
-            sFY[2 * S::FY_SZ + idx] = f.mom_n;   // Momy through y faces
-            sFY[3 * S::FY_SZ + idx] = f.energy;
-            sFY[4 * S::FY_SZ + idx] = f.bt;      // Bx flux through y faces
-            if (want_mask) {
-                int i = fx - 1, j = fy - 1;
-                if (i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny && bits)
-                    atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
-            }
+            FaceState a{L[F_RHO], dir ? L[F_VY] : L[F_VX], dir ? L[F_VX] : L[F_VY], L[F_P],
+                        dir ? L[F_BY] : L[F_BX], dir ? L[F_BX] : L[F_BY]};
+            FaceState bR{R[F_RHO], dir ? R[F_VY] : R[F_VX], dir ? R[F_VX] : R[F_VY], R[F_P],
+                         dir ? R[F_BY] : R[F_BX], dir ? R[F_BX] : R[F_BY]};
+            Flux5 f = riemann<SOLVER>(a, bR, gam);
+            double *sF = dir ? sFY : sFX;
+            const int fsz = dir ? S::FY_SZ : S::FX_SZ;
+            const int idx = dir ? fy * S::FYW + fx : fy * S::FXW + fx;
+            sF[0 * fsz + idx] = f.mass;
+            sF[1 * fsz + idx] = dir ? f.mom_t : f.mom_n;   // Momx
+            sF[2 * fsz + idx] = dir ? f.mom_n : f.mom_t;   // Momy
+            sF[3 * fsz + idx] = f.energy;
+            sF[4 * fsz + idx] = f.bt;                      // By flux (x faces) / Bx flux (y faces)
+            if (want_mask && owned && bits)
+                atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
         }
     }
     __syncthreads();

@@ -136,7 +136,7 @@ class Grid:
 
     def autotune(self, reps: int = 3) -> tuple[int, list[float]]:
         best = C.c_int()
-        ms = (C.c_double * 3)()
+        ms = (C.c_double * 4)()
         check(self._L.chimera_ctx_autotune(self._h, int(reps), C.byref(best), ms), "autotune")
         return best.value, list(ms)
 

@@ -53,10 +53,13 @@ def test_batch_reference_dt_policy_and_autotune(gpu_lib, orc):
         assert np.array_equal(outs[m], cpu)
 
 
-@pytest.mark.parametrize("shape", [1, 2])
-def test_single_grid_tile_shapes(gpu_lib, shape):
+@pytest.mark.parametrize("shape,scheme", [(1, (3, 1, 1, 1)), (2, (3, 1, 1, 1)),
+                                          (3, (2, 1, 1, 1)), (3, (1, 2, 1, 1))])
+def test_single_grid_tile_shapes(gpu_lib, shape, scheme):
+    """Every tile shape gives the 32 x 8 result bit for bit (72^2: partial tiles; the
+    32 x 32 shape on a grid smaller than three tiles per side)."""
     from chimera_amd.solver import Grid
-    prob = problems.rotor(72, scheme=(3, 1, 1, 1))
+    prob = problems.rotor(72, scheme=scheme)
     with Grid.from_problem(prob) as g:
         g.steps(3, dt_mode=1)
         ref = g.download()
@@ -73,5 +76,16 @@ def test_single_grid_autotune_keeps_state(gpu_lib):
     prob = problems.orszag_tang(128)
     with Grid.from_problem(prob) as g:
         best, ms = g.autotune(reps=2)
-        assert 0 <= best <= 2 and all(t > 0 for t in ms)
+        assert 0 <= best <= 3 and all(t > 0 for t in ms)
         assert np.array_equal(g.download(), prob.state)
+
+
+def test_large_tile_unavailable_for_hlld(gpu_lib):
+    from chimera_amd._lib import ChimeraError
+    from chimera_amd.solver import Grid
+    prob = problems.orszag_tang(64, scheme=(3, 2, 1, 1))
+    with Grid.from_problem(prob) as g:
+        with pytest.raises(ChimeraError):
+            g.set_tile_shape(3)
+        best, ms = g.autotune(reps=1)
+        assert best != 3 and ms[3] == -1.0
