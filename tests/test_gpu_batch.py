@@ -89,3 +89,23 @@ def test_large_tile_unavailable_for_hlld(gpu_lib):
             g.set_tile_shape(3)
         best, ms = g.autotune(reps=1)
         assert best != 3 and ms[3] == -1.0
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_cfl_after_reupload_and_set_time(gpu_lib, orc, graph):
+    """CFL steps after a re-upload and a set_time reduce the new state (eager and
+    graph replay): the run equals the oracle's on the uploaded state bit for bit."""
+    from chimera_amd.solver import Grid
+    a = problems.orszag_tang(64, scheme=(1, 2, 1, 1))
+    b = problems.rotor(64, scheme=(1, 2, 1, 1))
+    b = problems.Problem("rotor_as_ot", b.state, a.dx, a.gamma, a.bc, a.scheme)
+    with Grid.from_problem(a) as g:
+        g.steps(3, dt_mode=1, graph=graph)
+        g.upload(b.state)
+        g.steps(3, dt_mode=1, graph=graph)
+        g.set_time(0.0, 1)
+        g.steps(2, dt_mode=1, graph=graph)
+        out = g.download()
+    ref = b.state.copy()
+    orc.run(ref, b.dx, b.gamma, b.bc, list(b.scheme), 5, dt_mode=1)
+    assert np.array_equal(out, ref)
