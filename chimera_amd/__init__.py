@@ -4,7 +4,12 @@ Modules:
   _lib      ctypes binding of libchimera_hip.so (include/chimera_hip.h)
   solver    resident single-grid driver (Grid)
   dropin    per-routine drop-ins with the reference's argument meaning
-  problems  host-side initial conditions (OT, rotor, KH, field loop, GRF)
+  problems  initial conditions (OT, rotor, KH, field loops, GRF ensemble members and
+            the reference's random problem on the GPU)
+  batch     batched GRF ensemble
+  adjoint   hand-written adjoint behind torch.autograd
+  h5io      snapshot files in the reference's HDF5 layout (libchimera_h5.so)
+  cli       the reference's command line on the resident path
 """
 from . import problems  # noqa: F401
 

@@ -1,9 +1,11 @@
-"""Build libchimera_hip.so (gfx950) in-tree with hipcc.
+"""Build libchimera_hip.so (gfx950) in-tree with hipcc, and libchimera_h5.so.
 
 Every HIP source under chimera_amd/csrc is compiled for gfx950 with
 -ffp-contract=off (no FMA contraction: the results must match the reference
 Fortran bit for bit) and with packed-FP32 instruction selection disabled, then
-linked into chimera_amd/libchimera_hip.so.
+linked into chimera_amd/libchimera_hip.so.  The snapshot writer (csrc/h5io.c) is
+plain C over the HDF5 C library and goes into its own chimera_amd/libchimera_h5.so,
+so the solver library does not depend on HDF5.
 """
 from __future__ import annotations
 
@@ -17,6 +19,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libchimera_hip.so"
+LIB_H5 = PKG / "libchimera_h5.so"
+HDF5_PREFIX = Path(os.environ.get("CHIMERA_HDF5_PREFIX", "/opt/conda"))
 OBJ = PKG / "_obj"
 
 HIPCC_FLAGS = [
@@ -65,5 +69,25 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+def build_h5(verbose: bool = False) -> Path | None:
+    """Build the snapshot writer against the HDF5 C library; None when HDF5 is absent."""
+    inc, lib = HDF5_PREFIX / "include", HDF5_PREFIX / "lib"
+    if not (inc / "hdf5.h").exists():
+        if verbose:
+            print(f"HDF5 not found under {HDF5_PREFIX}; libchimera_h5.so not built", file=sys.stderr)
+        return None
+    tmp = LIB_H5.with_suffix(".so.tmp")
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-Wall", f"-I{inc}", "-o", str(tmp),
+           str(CSRC / "h5io.c"), f"-L{lib}", "-lhdf5", f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gcc failed on h5io.c:\n{r.stderr}")
+    os.replace(tmp, LIB_H5)
+    if verbose:
+        print(f"built {LIB_H5}", file=sys.stderr)
+    return LIB_H5
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_h5(verbose=True)

@@ -172,6 +172,46 @@ def field_loop(nx: int = 512, ny: int = 256, Az_scale=None, scheme=(2, 2, 1, 1),
     return (prob, Az) if return_az else prob
 
 
+def field_loop_unit(n: int, scheme=(1, 2, 1, 1)) -> Problem:
+    """The reference's field-loop problem (problem type 3), src/mhd_init.f90:360-391:
+    unit box, loop of radius 0.3 and amplitude 1e-3 centred at (0.5, 0.5), Az at cell
+    centres, rho = p = 1, v = (-2, -1), gamma = 5/3, periodic."""
+    dx = 1.0 / n
+    x = _centres(n, dx)
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2)
+    Az = np.where(r < 0.3, 1.0e-3 * (0.3 - r), 0.0)
+    bx, by = curl_2d(Az, dx, PERIODIC)
+    Bx, By = face_to_cell_B(bx, by, PERIODIC)
+    rho = np.ones((n, n))
+    vx = np.full((n, n), -2.0)
+    vy = np.full((n, n), -1.0)
+    P = 1.0 + 0.5 * (Bx * Bx + By * By)
+    gamma = 5.0 / 3.0
+    st = _pack(*prim_to_cons(rho, vx, vy, P, Bx, By, gamma, dx * dx), bx, by)
+    return Problem("field_loop_unit", st, dx, gamma, PERIODIC, tuple(scheme))
+
+
+def realized_parameters(prob: Problem) -> tuple[float, float]:
+    """(M_s, beta) of an initial state as the reference reports them for its random
+    problem (src/mhd_init.f90:833-841): rms |v| over the sound speed of the mean state,
+    and 2 <p> / B_rms^2 (p thermal, cell B)."""
+    st = np.asarray(prob.state)
+    vol = prob.dx * prob.dx
+    bx, by = st[4], st[5]
+    Bx, By = face_to_cell_B(bx, by, prob.bc)
+    rho = st[0] / vol
+    vx, vy = st[1] / st[0], st[2] / st[0]
+    half_b2 = 0.5 * (Bx * Bx + By * By)
+    p = (st[3] / vol - 0.5 * rho * (vx * vx + vy * vy) - half_b2) * (prob.gamma - 1.0)
+    n = rho.size
+    v_rms = np.sqrt(np.sum(vx * vx + vy * vy) / n)
+    b_rms = np.sqrt(np.sum(Bx * Bx + By * By) / n)
+    p_avg = np.sum(p) / n
+    c_s = np.sqrt(prob.gamma * p_avg / (np.sum(rho) / n))
+    return float(v_rms / (c_s + 1.0e-16)), float(2.0 * p_avg / (b_rms * b_rms + 1.0e-16))
+
+
 def field_loop_from_az(Az: np.ndarray, dx: float, scheme=(2, 2, 1, 1), hydro=None) -> Problem:
     """Field-loop state for a given corner potential Az (periodic).  ``hydro``
     (optional (rho, vx, vy, p_thermal) arrays) replaces the uniform rho = p = 1,
@@ -331,6 +371,135 @@ def grf_state_from_noise(w, n: int, params: dict):
     M = torch.full_like(vx, vol)
     E = (1.0 / (gamma - 1.0) + 0.5 * (vx * vx + vy * vy) + half_b2) * vol
     return torch.stack([M, M * vx, M * vy, E, bx, by]).contiguous()
+
+
+# ---- the reference's random problem (problem type 5) ----------------------------
+
+def grf_problem_uniforms(n: int, seed: int) -> dict:
+    """The uniform draws of the reference's random problem, in the order it consumes
+    them (src/mhd_init.f90:504-710): three for gamma, M_s and beta, one for the RBF
+    length, then for each of the four fields two per Fourier mode, ky outer and kx
+    inner ((n, n, 2) as [ky, kx, (u1, u2)]).  numpy's PCG64 seeded with ``seed``
+    stands in for the Fortran compiler's random_number, which no other build
+    reproduces."""
+    rng = np.random.default_rng(seed)
+    return {"params": rng.random(3), "ell": rng.random(),
+            "fields": [rng.random((n, n, 2)) for _ in range(4)]}
+
+
+def grf_problem_parameters(u3, target_M_s: float = -1.0, target_beta: float = -1.0) -> dict:
+    """sample_physics_parameters (src/mhd_init.f90:504-586): gamma ~ U(1.2, 1.7),
+    log10 M_s ~ U(log10 0.2, log10 2), log10 beta ~ U(log10 0.5, log10 5), overridden
+    by positive targets; rho = p = 1 give c_s = sqrt(gamma), V_0 = M_s c_s and
+    B_0 = sqrt(2 / beta)."""
+    gamma = 1.2 + 0.5 * u3[0]
+    if target_M_s > 0.0:
+        m_s = target_M_s
+    else:
+        m_s = 10.0 ** (np.log10(0.2) + (np.log10(2.0) - np.log10(0.2)) * u3[1])
+        m_s = min(max(m_s, 0.2), 2.0)
+    if target_beta > 0.0:
+        beta = target_beta
+    else:
+        beta = 10.0 ** (np.log10(0.5) + (np.log10(5.0) - np.log10(0.5)) * u3[2])
+        beta = min(max(beta, 0.5), 5.0)
+    c_s = np.sqrt(gamma)
+    return {"gamma": float(gamma), "M_s": float(m_s), "beta": float(beta),
+            "V_0": float(m_s * c_s), "B_0": float(np.sqrt(2.0 / beta))}
+
+
+def grf_spectral_field(u, ell: float, device=None):
+    """generate_GRF (src/mhd_init.f90:617-710) with an inverse FFT in place of the
+    direct O(n^4) sum: complex Gaussian noise from Box-Muller on the uniforms
+    u = [ky, kx, (u1, u2)] (u1 floored at 1e-15), amplitude exp(-ell^2 |k|^2 / 2) with
+    k = 2 pi (signed index), imaginary parts of the (0,0), (n/2,0), (0,n/2) and
+    (n/2,n/2) modes cleared, real part of the inverse DFT (exponent +i, 1/n^2), mean
+    removed.  Returns an (n, n) float64 tensor [iy, ix] on ``device``."""
+    import torch
+    u = torch.as_tensor(u, dtype=torch.float64, device=device)
+    n = u.shape[0]
+    dev = u.device
+    u1 = torch.clamp(u[..., 0], min=1.0e-15)
+    r = torch.sqrt(-2.0 * torch.log(u1))
+    ph = TWO_PI * u[..., 1]
+    kk = torch.arange(n, device=dev, dtype=torch.float64)
+    kk = torch.where(kk > n // 2, kk - n, kk) * TWO_PI
+    S = torch.exp(-0.5 * ell * ell * (kk[None, :] ** 2 + kk[:, None] ** 2))
+    re, im = S * r * torch.cos(ph), S * r * torch.sin(ph)
+    h = n // 2
+    for (ky, kx) in ((0, 0), (0, h), (h, 0), (h, h)):
+        im[ky, kx] = 0.0
+    f = torch.fft.ifft2(torch.complex(re, im)).real
+    return f - f.sum() / (n * n)
+
+
+def grf_problem(n: int, seed: int, target_M_s: float = -1.0, target_beta: float = -1.0,
+                scheme=(1, 2, 1, 1), device=None) -> tuple[Problem, dict]:
+    """The reference's random problem, setup_Monte_Carlo and setup_GRF_fields
+    (src/mhd_init.f90:461-486, 731-860), on ``device`` (torch; the FFTs run there):
+
+      psi, A, g_rho, g_p = four GRFs with one RBF length ell = 0.15 + 0.1 U;
+      v = 0.05 (d psi/dy, -d psi/dx) (centred) scaled to rms V_0;
+      Az = 0.001 A, face B its backward curl, cell B the face average, both scaled
+      to rms |B| = B_0;
+      rho = max(1 + 0.1 g_rho / max|g_rho|, 0.1), p = max(1 + 0.05 g_p / max|g_p|, 0.1);
+      v scaled down (B is not) so that max(c_f + |v|) <= 3;
+      realized M_s = v_rms / c_s(mean p, mean rho), beta = 2 <p> / B_rms^2.
+
+    Returns (Problem with a float64 tensor state (6, n, n), info) where info holds the
+    sampled and the realized parameters.  The reference's fallback for a vanishing
+    potential (rms B <= 1e-10) is not representable with face B and is rejected."""
+    import torch
+    u = grf_problem_uniforms(n, seed)
+    par = grf_problem_parameters(u["params"], target_M_s, target_beta)
+    gamma, V_0, B_0 = par["gamma"], par["V_0"], par["B_0"]
+    ell = 0.15 + 0.1 * u["ell"]
+    psi, A, g_rho, g_p = (grf_spectral_field(w, ell, device) for w in u["fields"])
+    dx = 1.0 / n
+    nn = float(n * n)
+    vx = 0.05 * (torch.roll(psi, -1, 0) - torch.roll(psi, 1, 0)) / (2.0 * dx)
+    vy = -0.05 * (torch.roll(psi, -1, 1) - torch.roll(psi, 1, 1)) / (2.0 * dx)
+    v_rms = torch.sqrt(torch.sum(vx * vx + vy * vy) / nn)
+    if float(v_rms) > 1.0e-10:
+        vx, vy = vx * (V_0 / v_rms), vy * (V_0 / v_rms)
+    Az = 0.001 * A
+    inv_dx = 1.0 / dx
+    bx = (Az - torch.roll(Az, 1, 0)) * inv_dx
+    by = -((Az - torch.roll(Az, 1, 1)) * inv_dx)
+    Bx = 0.5 * (bx + torch.roll(bx, 1, 1))
+    By = 0.5 * (by + torch.roll(by, 1, 0))
+    b_rms = torch.sqrt(torch.sum(Bx * Bx + By * By) / nn)
+    if not float(b_rms) > 1.0e-10:
+        raise ValueError("GRF potential vanishes (rms B <= 1e-10)")
+    sc = B_0 / b_rms
+    bx, by, Bx, By = bx * sc, by * sc, Bx * sc, By * sc
+    for g in (g_rho, g_p):
+        m = torch.max(torch.abs(g))
+        if float(m) > 1.0e-10:
+            g /= m
+    rho = torch.clamp(1.0 + 0.1 * g_rho, min=0.1)
+    p = torch.clamp(1.0 + 0.05 * g_p, min=0.1)
+    b2 = Bx * Bx + By * By
+    c_s = torch.sqrt(gamma * p / rho)
+    c_a = torch.sqrt(b2 / rho)
+    total_max = float(torch.max(torch.sqrt(c_s * c_s + c_a * c_a) + torch.sqrt(vx * vx + vy * vy)))
+    if total_max > 3.0:
+        vx, vy = vx * (3.0 / total_max), vy * (3.0 / total_max)
+    v_rms = float(torch.sqrt(torch.sum(vx * vx + vy * vy) / nn))
+    b_rms = float(torch.sqrt(torch.sum(b2) / nn))
+    p_avg = float(torch.sum(p) / nn)
+    c_s_avg = float(np.sqrt(gamma * p_avg / (float(torch.sum(rho)) / nn)))
+    info = dict(par, ell=ell, seed=seed, M_s_actual=v_rms / (c_s_avg + 1.0e-16),
+                beta_actual=2.0 * p_avg / (b_rms * b_rms + 1.0e-16), cf_max=total_max)
+    # get_conserved with total pressure (src/mhd_change_states.f90:16-58)
+    vol = dx * dx
+    r = torch.clamp(rho, min=RHO_FLOOR)
+    ptot = torch.clamp(p + 0.5 * b2, min=P_FLOOR)
+    half_b2 = 0.5 * b2
+    M = r * vol
+    E = ((ptot - half_b2) / (gamma - 1.0) + 0.5 * r * (vx * vx + vy * vy) + half_b2) * vol
+    st = torch.stack([M, M * vx, M * vy, E, bx, by]).contiguous()
+    return Problem(f"grf_problem_{n}_{seed}", st, dx, gamma, PERIODIC, tuple(scheme), info), info
 
 
 def grf_member_device(n: int, seed: int, scheme=(1, 2, 1, 1), device="cuda", **kw):

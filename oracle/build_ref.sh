@@ -19,6 +19,8 @@
 # in place of the five reference modules:
 #   oracle/_ref/libchimera_dropin_s<solver>_l<limiter>.so
 # -- the same harness and the reference's mhd_config, every routine on the GPU.
+# With chimera_amd/libchimera_h5.so also present, the reference's main program is
+# linked the same way (see the end of this script).
 set -euo pipefail
 
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
@@ -78,5 +80,44 @@ for s in 1 2 3; do
     fi
   done
 done
+
+# The reference's own program (main.f90, mhd_init.f90 and its mhd_config, unmodified):
+#   oracle/_ref/chimera_ref_main     the reference modules, with only mhd_write_h5
+#                                    swapped for the drop-in writer (the reference's
+#                                    needs the gfortran HDF5 module); runs on the CPU
+#   oracle/_ref/chimera_dropin_main  the drop-in modules and writer: every hot-path
+#                                    routine on the GPU
+# both with the reference CLI: <problem_type> <N> <seed> [output_path] [h5_filename] ...
+LIBH5="$HERE/../chimera_amd/libchimera_h5.so"
+if [ -f "$LIBH5" ]; then
+  R="$TMP/refmain"
+  mkdir -p "$R"
+  robjs=()
+  for m in "$SRC/mhd_config.f90" "$SRC/mhd_field_ops.f90" "$SRC/mhd_change_states.f90" \
+           "$SRC/mhd_derivatives.f90" "$SRC/mhd_bc.f90" "$SRC/mhd_flux.f90" \
+           "$SHIMS/mhd_write_h5.f90" "$SRC/mhd_init.f90" "$SRC/main.f90"; do
+    o="$R/$(basename "$m" .f90).o"
+    $FC $FFLAGS -module-dir "$R" -I "$R" -c "$m" -o "$o"
+    robjs+=("$o")
+  done
+  $FC -fopenmp -o "$OUT/chimera_ref_main" "${robjs[@]}" \
+      -L"$(dirname "$LIBH5")" -lchimera_h5 -Wl,-rpath,'$ORIGIN/../../chimera_amd'
+fi
+if [ -f "$LIBHIP" ] && [ -f "$LIBH5" ]; then
+  M="$TMP/main"
+  mkdir -p "$M"
+  mobjs=()
+  for m in "$SRC/mhd_config.f90" "$SHIMS/chimera_c.f90" "$SHIMS/mhd_field_ops.f90" \
+           "$SHIMS/mhd_change_states.f90" "$SHIMS/mhd_derivatives.f90" "$SHIMS/mhd_bc.f90" \
+           "$SHIMS/mhd_flux.f90" "$SHIMS/mhd_write_h5.f90" "$SRC/mhd_init.f90" "$SRC/main.f90"; do
+    o="$M/$(basename "$m" .f90).o"
+    $FC $FFLAGS -module-dir "$M" -I "$M" -c "$m" -o "$o"
+    mobjs+=("$o")
+  done
+  $FC -fopenmp -o "$OUT/chimera_dropin_main" "${mobjs[@]}" \
+      -L"$(dirname "$LIBHIP")" -lchimera_hip -lchimera_h5 -Wl,-rpath,'$ORIGIN/../../chimera_amd'
+fi
 echo "build_ref.sh: built $(ls "$OUT"/libchimera_ref_*.so | wc -l) reference and" \
-     "$(ls "$OUT"/libchimera_dropin_*.so 2>/dev/null | wc -l) drop-in variants into $OUT"
+     "$(ls "$OUT"/libchimera_dropin_*.so 2>/dev/null | wc -l) drop-in variants into $OUT" \
+     "$( [ -f "$OUT/chimera_ref_main" ] && echo '+ chimera_ref_main')" \
+     "$( [ -f "$OUT/chimera_dropin_main" ] && echo '+ chimera_dropin_main')"

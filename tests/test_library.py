@@ -14,9 +14,13 @@ ROOT = Path(__file__).resolve().parent.parent
 HEADERS = sorted((ROOT / "include").glob("*.h"))
 
 
-def declared_functions() -> set[str]:
+# the library that implements each header's declarations
+LIBRARY_OF = {"chimera_hip.h": "libchimera_hip.so", "chimera_h5.h": "libchimera_h5.so"}
+
+
+def declared_functions(headers=None) -> set[str]:
     names = set()
-    for h in HEADERS:
+    for h in headers or HEADERS:
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(chimera_\w+)\s*\(", text, re.M):
             names.add(m.group(1))
@@ -34,15 +38,23 @@ def test_headers_declare_api():
     assert {"chimera_compute_fluxes", "chimera_ctx_create", "chimera_ctx_steps"} <= names
 
 
-def test_library_exports_every_declared_symbol(L):
-    lib = ROOT / "chimera_amd" / "libchimera_hip.so"
+def test_every_header_has_a_library():
+    assert {h.name for h in HEADERS} == set(LIBRARY_OF)
+
+
+@pytest.mark.parametrize("header", sorted(LIBRARY_OF))
+def test_library_exports_every_declared_symbol(header):
+    lib = ROOT / "chimera_amd" / LIBRARY_OF[header]
     out = subprocess.run(["nm", "-D", "--defined-only", str(lib)], capture_output=True, text=True,
                          check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
-    missing = declared_functions() - exported
+    declared = declared_functions([ROOT / "include" / header])
+    assert declared
+    missing = declared - exported
     assert not missing, f"declared but not exported: {sorted(missing)}"
-    for name in declared_functions():
-        getattr(L, name)          # resolves through dlsym
+    handle = C.CDLL(str(lib))
+    for name in declared:
+        getattr(handle, name)     # resolves through dlsym
 
 
 def test_invalid_arguments_rejected_on_host(L):
