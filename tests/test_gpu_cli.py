@@ -50,13 +50,14 @@ def test_cli_matches_reference_main(problem_type, n, cli_run, ref_main_run):
     da, db, worst = _compare_files(ours, ref)
     print(f"problem {problem_type} N={n}: {len(da['rho'])} snapshots, worst rel L1 {worst:.3e}")
     assert da["parameters"] == db["parameters"]
-    # progress lines: same percentages and times; avg |div B| both at round-off
+    # progress lines: same percentages and times; avg |div B| sums the same per-cell
+    # values in another order, so the printed 5 digits agree up to the last one
     po = _progress(cli_run.stdout[(problem_type, n, 0, None)])
     pr = _progress(ref_main_run.stdout[(problem_type, n, 0, None)])
     assert len(po) == len(pr) == 100
     assert [p[:2] for p in po] == [p[:2] for p in pr]
     for p, q in zip(po, pr):
-        assert float(p[2]) < 1e-12 and float(q[2]) < 1e-12
+        assert abs(float(p[2]) - float(q[2])) <= 2e-4 * abs(float(q[2])), (p, q)
 
 
 @pytest.mark.skipif(not DROPIN_MAIN.exists(), reason="oracle/_ref/chimera_dropin_main is not built")
