@@ -176,6 +176,15 @@ def test_cli_usage_error_stops_like_the_reference():
     assert "ERROR: problem_type must be 1-5, got: '9'" in r.stdout
 
 
+def test_executable_wrapper(tmp_path):
+    """chimera_amd/bin/chimera, linked where the reference's drivers expect ./chimera."""
+    link = tmp_path / "chimera"
+    link.symlink_to(ROOT / "chimera_amd" / "bin" / "chimera")
+    r = subprocess.run([str(link), "1", "2", "0"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "ERROR: N must be an integer >= 4, got: '2'" in r.stdout
+
+
 @pytest.mark.skipif(not REF_MAIN.exists(), reason="oracle/_ref/chimera_ref_main is not built")
 def test_cli_messages_match_reference_main():
     for argv in (["1", "32"], ["6", "32", "0"], ["1", "2", "0"], ["1", "32", "q"]):
