@@ -132,6 +132,7 @@ def _declare(L: C.CDLL) -> None:
         "chimera_adj_forward_step": [vp, C.POINTER(vp), d, C.POINTER(vp)],
         "chimera_adj_step_vjp": [vp, C.POINTER(vp), d, C.POINTER(vp), C.POINTER(vp)],
         "chimera_adj_segment_vjp": [vp, C.POINTER(vp), c_dp, i, C.POINTER(vp)],
+        "chimera_adj_set_fused": [vp, i],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

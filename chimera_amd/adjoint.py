@@ -40,6 +40,11 @@ class AdjointStepper:
               "chimera_adj_create")
         self._h = h
 
+    def set_fused(self, fused: bool) -> None:
+        """Fused LDS-tiled reverse kernels (default) or the stored-intermediate ones;
+        both give the same bits."""
+        check(self._L.chimera_adj_set_fused(self._h, int(bool(fused))), "chimera_adj_set_fused")
+
     def forward_step(self, x: torch.Tensor, dt: float, out: torch.Tensor | None = None):
         out = torch.empty_like(x) if out is None else out
         torch.cuda.synchronize()

@@ -53,8 +53,9 @@ __device__ __forceinline__ int wrap_or_clamp(int g, int n, int periodic) {
 // ---- forward recompute: the fused stage kernel, keeping the intermediates -----------
 
 constexpr int ATX = 32, ATY = 8;
+constexpr int RTX = 16, RTY = 16;   // tiles of the fused reverse kernels
 
-template <int SOLVER, int LIMITER>
+template <int SOLVER, int LIMITER, bool STORE>
 __global__ __launch_bounds__(STAGE_THREADS) void adj_stage_kernel(GridView g, StageStore st,
                                                                  int stage, int use_lim, int mood,
                                                                  int tiles_x, int ntiles) {
@@ -62,8 +63,8 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_stage_kernel(GridView g, St
     const int b = blockIdx.x, nb = gridDim.x;  // nb is a multiple of 8
     const int tile = (b % 8) * (nb / 8) + b / 8;
     if (tile >= ntiles) return;
-    stage_tile<SOLVER, LIMITER, ATX, ATY, true>(g, stage, use_lim != 0, mood != 0, tile % tiles_x,
-                                                tile / tiles_x, lds, &st);
+    stage_tile<SOLVER, LIMITER, ATX, ATY, STORE>(g, stage, use_lim != 0, mood != 0, tile % tiles_x,
+                                                 tile / tiles_x, lds, STORE ? &st : nullptr);
 }
 
 // ---- reverse -------------------------------------------------------------------
@@ -356,6 +357,510 @@ __global__ void ar_bavg(AdjGrid G, RevBufs R, double *const *lin) {
     }
 }
 
+// ---- fused reverse -----------------------------------------------------------------
+//
+// The same reverse in two LDS-tiled kernels that recompute the forward stage instead of
+// reading stored intermediates.
+//
+// ar_tile_a (one TX x TY tile of owned cells per workgroup) replaces the STORE
+// recompute, ar_cell_tail, ar_corner and ar_faces: it loads the primitives of cells
+// [-2, TX+2] x [-2, TY+2], computes the fluxes of the faces of cells [0, TX] x [0, TY]
+// (forward phase 2), the flux update, floors, outflow correction and blend of those
+// cells with the reverse of the tail (lam of Up into LDS), lam of -Ez at the corners
+// around the owned faces, and finally, per owned cell, the face-state recompute, the
+// Riemann VJP and the reverse of the face construction of its x and y faces.  It
+// writes what the slope / primitive reverse needs (stage primitives, lam of the
+// face-state candidates, lam of the cell resets, of the outflow primitives and of the
+// cell B) and the update's and CT's part of lam_in; lam_u0 is accumulated.
+//
+// ar_tile_b replaces ar_slopes, ar_prims and ar_bavg: field by field it gathers the
+// slope cotangents of cells [-1, TX+1] x [-1, TY+1] into LDS, sums each cell's own and
+// its neighbours' parts for cells [0, TX] x [0, TY], then reverses cons_to_prim and the
+// face -> cell B average.  Every value is computed with the arithmetic, order and
+// branch of the unfused kernels, so both paths give the same bits.
+
+template <int TX, int TY>
+struct RevShapeA {
+    static constexpr int PW = TX + 5, PH = TY + 5, P_SZ = PW * PH;        // cells [-2, T+2]
+    static constexpr int FXW = TX + 2, FXH = TY + 1, FX_SZ = FXW * FXH;   // x faces: left cell [-1, TX] x [0, TY]
+    static constexpr int FYW = TX + 1, FYH = TY + 2, FY_SZ = FYW * FYH;   // y faces: lower cell [0, TX] x [-1, TY]
+    static constexpr int CW = TX + 1, CH = TY + 1, C_SZ = CW * CH;        // lam Up: cells [0, TX] x [0, TY]
+    static constexpr int EW = TX + 1, EH = TY + 1, E_SZ = EW * EH;        // lam -Ez: corners [-1, TX) x [-1, TY)
+    static constexpr int LDS_DOUBLES = 6 * P_SZ + 4 * FX_SZ + 4 * FY_SZ + 4 * C_SZ + E_SZ;
+};
+
+// Face states of one face from the primitives in LDS exactly as the forward stage
+// builds them (slopes of both cells, MUSCL, MOOD, rho/P floor, thermal check).  b0
+// points at the left / lower cell's rho; sA / sT are the strides along / across the
+// face normal; returns the MOOD and thermal bits and the floor bits at the positions
+// of the stored words (direction dir).
+template <int LIMITER>
+__device__ __forceinline__ unsigned long long face_pair(const double *b0, int psz, int sA, int sT,
+                                                        bool last, bool mood, bool use_lim,
+                                                        double inv_dx, double inv_2dx, double hdx,
+                                                        int dir, double L[6], double R[6]) {
+    double C[6];
+    unsigned bits = 0, fbits = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const double *b = b0 + q * psz;
+        const double m = b[-sA], c = b[0], p = b[sA], pp = b[2 * sA];
+        C[q] = c;
+        const double sl = cell_slope<LIMITER>(m, c, p, use_lim, inv_dx, inv_2dx);
+        const double sr = cell_slope<LIMITER>(c, p, pp, use_lim, inv_dx, inv_2dx);
+        unsigned fb;
+        bits |= face_states(m, c, p, b[sT], b[-sT], sl, sr, last, mood, q == F_RHO || q == F_P,
+                            hdx, L[q], R[q], fb)
+                << (4 * q + 2 * dir);
+        if (q == F_RHO) fbits |= fb << (2 * dir);
+        if (q == F_P) fbits |= fb << (4 + 2 * dir);
+    }
+    if (thermal_reset(L, C)) bits |= 1u << (24 + 2 * dir);
+    if (thermal_reset(R, C)) bits |= 1u << (25 + 2 * dir);
+    return (unsigned long long)bits | ((unsigned long long)fbits << 32);
+}
+
+template <int SOLVER, int LIMITER, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, StageBufs S, RevBufs R,
+                                                   const double *const *lout, double *const *lin,
+                                                   double *const *lU0, int stage, double dt,
+                                                   int tiles_x, int ntiles) {
+    using A = RevShapeA<TX, TY>;
+    constexpr int NT = TX * TY;
+    __shared__ double lds[A::LDS_DOUBLES];
+    const int b = blockIdx.x, nb = gridDim.x;   // nb is a multiple of 8
+    const int tile = (b % 8) * (nb / 8) + b / 8;
+    if (tile >= ntiles) return;
+    const int tid = threadIdx.x;
+    const int nx = g.nx, ny = g.ny, px = g.px, py = g.py;
+    const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+    double *sP = lds;                          // [6][PH][PW]
+    double *sFX = sP + 6 * A::P_SZ;            // [4][FXH][FXW] mass, Momx, Momy, energy
+    double *sFY = sFX + 4 * A::FX_SZ;          // [4][FYH][FYW]
+    double *sLU = sFY + 4 * A::FY_SZ;          // [4][CH][CW]
+    double *sLE = sLU + 4 * A::C_SZ;           // [EH][EW]
+    const double dx = g.dx, vol = g.vol, gam = g.gamma;
+    const double inv_dx = 1.0 / dx, inv_2dx = 1.0 / (2.0 * dx), hdx = 0.5 * dx;
+    const bool use_lim = G.use_lim != 0, mood = G.mood != 0;
+
+    // ---- 1: primitives of cells [-2, TX+2] x [-2, TY+2] ------------------------------
+    for (int idx = tid; idx < A::P_SZ; idx += NT) {
+        const int lx = idx % A::PW, ly = idx / A::PW;
+        const int gi = map_index(x0 - 2 + lx, nx, px);
+        const int gj = map_index(y0 - 2 + ly, ny, py);
+        const int gim = gi > 0 ? gi - 1 : (px ? nx - 1 : 0);
+        const int gjm = gj > 0 ? gj - 1 : (py ? ny - 1 : 0);
+        const size_t k = (size_t)gi + (size_t)nx * gj;
+        const double Bx = 0.5 * (g.uin[4][k] + g.uin[4][(size_t)gim + (size_t)nx * gj]);
+        const double By = 0.5 * (g.uin[5][k] + g.uin[5][(size_t)gi + (size_t)nx * gjm]);
+        double rho, vx, vy, P;
+        cons_to_prim(g.uin[0][k], g.uin[1][k], g.uin[2][k], g.uin[3][k], Bx, By, gam, vol, rho, vx,
+                     vy, P);
+        sP[0 * A::P_SZ + idx] = rho;
+        sP[1 * A::P_SZ + idx] = vx;
+        sP[2 * A::P_SZ + idx] = vy;
+        sP[3 * A::P_SZ + idx] = P;
+        sP[4 * A::P_SZ + idx] = Bx;
+        sP[5 * A::P_SZ + idx] = By;
+        const int ci = lx - 2, cj = ly - 2;
+        if (ci >= 0 && ci < TX && cj >= 0 && cj < TY && x0 + ci < nx && y0 + cj < ny) {
+            const size_t ko = (size_t)(x0 + ci) + (size_t)nx * (y0 + cj);
+            S.P[0][ko] = rho;
+            S.P[1][ko] = vx;
+            S.P[2][ko] = vy;
+            S.P[3][ko] = P;
+            S.P[4][ko] = Bx;
+            S.P[5][ko] = By;
+        }
+    }
+    __syncthreads();
+
+    // ---- 2: fluxes of the faces of cells [0, TX] x [0, TY] ---------------------------
+    for (int f = tid; f < A::FX_SZ + A::FY_SZ; f += NT) {
+        const int dir = f >= A::FX_SZ;
+        int i, j;
+        if (!dir) {
+            i = f % A::FXW - 1;
+            j = f / A::FXW;
+        } else {
+            i = (f - A::FX_SZ) % A::FYW;
+            j = (f - A::FX_SZ) / A::FYW - 1;
+        }
+        const int lpos = (j + 2) * A::PW + (i + 2);
+        const bool last = dir ? (!py && map_index(y0 + j, ny, py) == ny - 1)
+                              : (!px && map_index(x0 + i, nx, px) == nx - 1);
+        double L[6], Rs[6];
+        (void)face_pair<LIMITER>(sP + lpos, A::P_SZ, dir ? A::PW : 1, dir ? 1 : A::PW, last, mood,
+                                 use_lim, inv_dx, inv_2dx, hdx, dir, L, Rs);
+        FaceState fa{L[F_RHO], dir ? L[F_VY] : L[F_VX], dir ? L[F_VX] : L[F_VY], L[F_P],
+                     dir ? L[F_BY] : L[F_BX], dir ? L[F_BX] : L[F_BY]};
+        FaceState fb{Rs[F_RHO], dir ? Rs[F_VY] : Rs[F_VX], dir ? Rs[F_VX] : Rs[F_VY], Rs[F_P],
+                     dir ? Rs[F_BY] : Rs[F_BX], dir ? Rs[F_BX] : Rs[F_BY]};
+        const Flux5 fl = riemann<SOLVER>(fa, fb, gam);
+        double *sF = dir ? sFY : sFX;
+        const int fsz = dir ? A::FY_SZ : A::FX_SZ;
+        const int idx = dir ? f - A::FX_SZ : f;
+        sF[0 * fsz + idx] = fl.mass;
+        sF[1 * fsz + idx] = dir ? fl.mom_t : fl.mom_n;
+        sF[2 * fsz + idx] = dir ? fl.mom_n : fl.mom_t;
+        sF[3 * fsz + idx] = fl.energy;
+    }
+    __syncthreads();
+
+    // ---- 3: forward tail and its reverse for cells [0, TX] x [0, TY] ------------------
+    const double dtdx = dt * dx;
+    const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
+    const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
+    for (int c = tid; c < A::C_SZ; c += NT) {
+        const int ci = c % A::CW, cj = c / A::CW;
+        const int gx = x0 + ci, gy = y0 + cj;
+        if ((!px && gx >= nx) || (!py && gy >= ny)) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sLU[q * A::C_SZ + c] = 0.0;
+            continue;
+        }
+        const int gi = map_index(gx, nx, px), gj = map_index(gy, ny, py);
+        const size_t k = (size_t)gi + (size_t)nx * gj;
+        const int xr = cj * A::FXW + (ci + 1);         // x face right of the cell
+        const int yr = (cj + 1) * A::FYW + ci;         // y face above the cell
+        const bool xlo_open = (gi == 0) && !px;
+        const bool ylo_open = (gj == 0) && !py;
+        double Up[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double v = g.uin[q][k];
+            const double fr = sFX[q * A::FX_SZ + xr];
+            if (xlo_open) v = v - dtdx * fr;
+            else v = v - dtdx * fr + dtdx * sFX[q * A::FX_SZ + xr - 1];
+            const double ft = sFY[q * A::FY_SZ + yr];
+            if (ylo_open) v = v - dtdx * ft;
+            else v = v - dtdx * ft + dtdx * sFY[q * A::FY_SZ + yr - A::FYW];
+            Up[q] = v;
+        }
+        const int pc = (cj + 2) * A::PW + (ci + 2);
+        double P6[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) P6[q] = sP[q * A::P_SZ + pc];
+        const double Bx = P6[4], By = P6[5];
+        double U3[4] = {Up[0], Up[1], Up[2], Up[3]};
+        conserved_floors(U3[0], U3[1], U3[2], U3[3], Bx, By, vol, gam);
+        if (xlo_open || ylo_open) {
+            const double rho = P6[0], vx = P6[1], vy = P6[2], P = P6[3];
+            const double h = 0.5 * (Bx * Bx + By * By);
+            const double en = (P - h) / (gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy) + h;
+            if (xlo_open) {
+                U3[0] = U3[0] + dtdx * rho * vx;
+                U3[1] = U3[1] + dtdx * (rho * (vx * vx) + P - Bx * Bx);
+                U3[2] = U3[2] + dtdx * (rho * vx * vy - Bx * By);
+                U3[3] = U3[3] + dtdx * ((en + P) * vx - Bx * (Bx * vx + By * vy));
+            }
+            if (ylo_open) {
+                U3[0] = U3[0] + dtdx * rho * vy;
+                U3[2] = U3[2] + dtdx * (rho * (vy * vy) + P - By * By);
+                U3[1] = U3[1] + dtdx * (rho * vy * vx - By * Bx);
+                U3[3] = U3[3] + dtdx * ((en + P) * vy - By * (By * vy + Bx * vx));
+            }
+        }
+        // reverse (as ar_cell_tail)
+        const bool owned = ci < TX && cj < TY && gx < nx && gy < ny;
+        double lBx = 0.0, lBy = 0.0;
+        double l3[4], lb[2];
+        if (stage > 1) {
+            double V[4], lo[4], lV[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                V[q] = c0 * g.u0[q][k] + c1 * U3[q];
+                lo[q] = lout[q][k];
+            }
+            floors_vjp(V, Bx, By, vol, gam, lo, lV, lBx, lBy);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (owned) lU0[q][k] += c0 * lV[q];
+                l3[q] = c1 * lV[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (owned) lU0[4 + q][k] += c0 * lout[4 + q][k];
+                lb[q] = c1 * lout[4 + q][k];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) l3[q] = lout[q][k];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) lb[q] = lout[4 + q][k];
+        }
+        double lPd[6] = {0, 0, 0, 0, 0, 0};
+        if (xlo_open) outflow_vjp(0, P6, gam, dtdx, l3, lPd);
+        if (ylo_open) outflow_vjp(1, P6, gam, dtdx, l3, lPd);
+        double lUp[4];
+        floors_vjp(Up, Bx, By, vol, gam, l3, lUp, lBx, lBy);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sLU[q * A::C_SZ + c] = lUp[q];
+        if (owned) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lin[q][k] = lUp[q];
+            lin[4][k] = lb[0];
+            lin[5][k] = lb[1];
+            R.lBc[0][k] = lBx;
+            R.lBc[1][k] = lBy;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) R.lPd[f][k] = lPd[f];
+        }
+    }
+
+    // ---- 4: lam of -Ez at corners [-1, TX) x [-1, TY) (as ar_corner) -----------------
+    {
+        const double s = dt / dx;
+        const double cb = stage > 1 ? c1 : 1.0;   // lam of the CT-updated face B
+        const double *lox = lout[4], *loy = lout[5];
+        for (int e = tid; e < A::E_SZ; e += NT) {
+            const int la = e % A::EW - 1, lb_ = e / A::EW - 1;
+            const int gx = x0 + la, gy = y0 + lb_;
+            if ((!px && (gx < 0 || gx >= nx)) || (!py && (gy < 0 || gy >= ny))) {
+                sLE[e] = 0.0;
+                continue;
+            }
+            const int a = map_index(gx, nx, px), bb = map_index(gy, ny, py);
+            const size_t k = (size_t)a + (size_t)nx * bb;
+            double acc = 0.0;
+            if (bb >= 1 || py) acc += s * (cb * lox[k]);
+            {
+                const int jn = bb + 1;
+                if (jn < ny) acc -= s * (cb * lox[k + nx]);
+                else if (py) acc -= s * (cb * lox[(size_t)a]);
+            }
+            if (!py) {
+                if (bb == 1) acc += s * (cb * lox[(size_t)a]);
+                if (bb == 0) acc -= s * (cb * lox[(size_t)a]);
+            }
+            if (a >= 1 || px) acc -= s * (cb * loy[k]);
+            {
+                const int in = a + 1;
+                if (in < nx) acc += s * (cb * loy[k + 1]);
+                else if (px) acc += s * (cb * loy[(size_t)nx * bb]);
+            }
+            if (!px) {
+                if (a == 1) acc -= s * (cb * loy[(size_t)nx * bb]);
+                if (a == 0) acc += s * (cb * loy[(size_t)nx * bb]);
+            }
+            sLE[e] = acc;
+        }
+    }
+    __syncthreads();
+
+    // ---- 5: owned faces (as ar_faces) -------------------------------------------------
+    {
+        const int ci = tid % TX, cj = tid / TX;
+        const int gi = x0 + ci, gj = y0 + cj;
+        if (gi < nx && gj < ny) {
+            const size_t k = (size_t)gi + (size_t)nx * gj;
+            auto lEz = [&](int a, int bb) { return -sLE[(bb + 1) * A::EW + (a + 1)]; };
+            const double *b0 = sP + (cj + 2) * A::PW + (ci + 2);
+            double lreset[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int dir = 0; dir < 2; ++dir) {
+                // the face states again, with this direction's bits (the x and y bits
+                // occupy disjoint positions of the stored words)
+                double L[6], Rr[6];
+                const bool lastf = dir ? (!py && gj == ny - 1) : (!px && gi == nx - 1);
+                const unsigned long long bits = face_pair<LIMITER>(
+                    b0, A::P_SZ, dir ? A::PW : 1, dir ? 1 : A::PW, lastf, mood, use_lim, inv_dx,
+                    inv_2dx, hdx, dir, L, Rr);
+                const bool has_next = dir ? (gj < ny - 1 || py) : (gi < nx - 1 || px);
+                const int cn = dir ? (cj + 1) * A::CW + ci : cj * A::CW + ci + 1;
+                const int cc = cj * A::CW + ci;
+                double lF[5];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    double v = -dtdx * sLU[q * A::C_SZ + cc];
+                    if (has_next) v += dtdx * sLU[q * A::C_SZ + cn];
+                    lF[q] = v;
+                }
+                if (!dir) {
+                    double v = -0.25 * lEz(ci, cj);
+                    if (gj >= 1 || py) v += -0.25 * lEz(ci, cj - 1);
+                    if (!py && gj == ny - 1) v += -0.25 * lEz(ci, cj);
+                    lF[4] = v;
+                } else {
+                    double v = 0.25 * lEz(ci, cj);
+                    if (gi >= 1 || px) v += 0.25 * lEz(ci - 1, cj);
+                    if (!px && gi == nx - 1) v += 0.25 * lEz(ci, cj);
+                    lF[4] = v;
+                }
+                const int fu = dir ? 2 : 1, fw = dir ? 1 : 2, fbn = dir ? 5 : 4, fbt = dir ? 4 : 5;
+                FaceState fa{L[0], L[fu], L[fw], L[3], L[fbn], L[fbt]};
+                FaceState fb{Rr[0], Rr[fu], Rr[fw], Rr[3], Rr[fbn], Rr[fbt]};
+                double lFr[5] = {lF[0], dir ? lF[2] : lF[1], dir ? lF[1] : lF[2], lF[3], lF[4]};
+                double lLr[6], lRr[6];
+                riemann_rev<SOLVER>(fa, fb, gam, lFr, lLr, lRr);
+                double lL[6], lR[6];
+                lL[0] = lLr[0]; lL[fu] = lLr[1]; lL[fw] = lLr[2]; lL[3] = lLr[3]; lL[fbn] = lLr[4]; lL[fbt] = lLr[5];
+                lR[0] = lRr[0]; lR[fu] = lRr[1]; lR[fw] = lRr[2]; lR[3] = lRr[3]; lR[fbn] = lRr[4]; lR[fbt] = lRr[5];
+                face_state_reverse(lL, dir ? 2 : 0, false, bits, lreset, dir ? R.lYL : R.lXL, k);
+                face_state_reverse(lR, dir ? 3 : 1, lastf, bits, lreset, dir ? R.lYR : R.lXR, k);
+            }
+#pragma unroll
+            for (int f = 0; f < 6; ++f) R.lPr[f][k] = lreset[f];
+        }
+    }
+}
+
+template <int TX, int TY>
+struct RevShapeB {
+    static constexpr int PW = TX + 5, PH = TY + 5, P_SZ = PW * PH;   // primitive: cells [-2, T+2]
+    static constexpr int GW = TX + 3, GH = TY + 3, G_SZ = GW * GH;   // slope parts: cells [-1, T+1]
+    static constexpr int CW = TX + 1, CH = TY + 1, C_SZ = CW * CH;   // cells [0, T]
+    static constexpr int LDS_DOUBLES = P_SZ + 5 * G_SZ + 2 * C_SZ;
+    static constexpr int CPT = (C_SZ + TX * TY - 1) / (TX * TY);     // cells [0, T] per thread
+};
+
+template <int LIMITER, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void ar_tile_b(AdjGrid G, const double *const *U, StageStore S,
+                                                   RevBufs R, double *const *lin, int tiles_x,
+                                                   int ntiles) {
+    using B = RevShapeB<TX, TY>;
+    constexpr int NT = TX * TY;
+    __shared__ double lds[B::LDS_DOUBLES];
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int tile = (b % 8) * (nb / 8) + b / 8;
+    if (tile >= ntiles) return;
+    const int tid = threadIdx.x;
+    const int nx = G.nx, ny = G.ny, px = G.px, py = G.py;
+    const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+    double *sA = lds;                       // one primitive, cells [-2, T+2]
+    double *sG = sA + B::P_SZ;              // gmx, gpx, gmy, gpy, lP of one field, cells [-1, T+1]
+    double *sB = sG + 5 * B::G_SZ;          // lam of cell B (total), cells [0, T]
+    const double hdx = 0.5 * G.dx, inv_dx = 1.0 / G.dx, inv_2dx = 1.0 / (2.0 * G.dx);
+    // valid global cell of a local position, or -1 outside a non-periodic grid
+    auto gcell = [&](int lx, int ly, int &gi, int &gj) -> bool {
+        const int ux = x0 + lx, uy = y0 + ly;
+        if ((!px && (ux < 0 || ux >= nx)) || (!py && (uy < 0 || uy >= ny))) return false;
+        gi = map_index(ux, nx, px);
+        gj = map_index(uy, ny, py);
+        return true;
+    };
+    double lp6[B::CPT][6];
+
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        // the field's stage primitive over cells [-2, T+2] (clamped like the forward)
+        for (int idx = tid; idx < B::P_SZ; idx += NT) {
+            const int gi = map_index(x0 - 2 + idx % B::PW, nx, px);
+            const int gj = map_index(y0 - 2 + idx / B::PW, ny, py);
+            sA[idx] = S.P[f][(size_t)gi + (size_t)nx * gj];
+        }
+        __syncthreads();
+        // slope parts of cells [-1, T+1] (as ar_slopes)
+        for (int idx = tid; idx < B::G_SZ; idx += NT) {
+            const int lx = idx % B::GW - 1, ly = idx / B::GW - 1;
+            int i, j;
+            if (!gcell(lx, ly, i, j)) {
+#pragma unroll
+                for (int r = 0; r < 5; ++r) sG[r * B::G_SZ + idx] = 0.0;
+                continue;
+            }
+            const size_t k = (size_t)i + (size_t)nx * j;
+            const bool hl = i > 0 || px, hb = j > 0 || py;
+            const size_t kl = i > 0 ? k - 1 : k + nx - 1;
+            const size_t kb = j > 0 ? k - nx : k + (size_t)nx * (ny - 1);
+            const bool cxm = !px && i == 0, cxp = !px && i == nx - 1;   // clamped neighbours
+            const bool cym = !py && j == 0, cyp = !py && j == ny - 1;
+            const double *a = sA + (ly + 2) * B::PW + (lx + 2);
+            const double am = cxm ? a[0] : a[-1], ap = cxp ? a[0] : a[1];
+            const double bm = cym ? a[0] : a[-B::PW], bp = cyp ? a[0] : a[B::PW];
+            const double xr = hl ? R.lXR[f][kl] : 0.0;
+            const double yr = hb ? R.lYR[f][kb] : 0.0;
+            const double lSx = hdx * (R.lXL[f][k] - xr);
+            const double lSy = hdx * (R.lYL[f][k] - yr);
+            double lp = R.lXL[f][k] + xr + R.lYL[f][k] + yr + R.lPr[f][k] + R.lPd[f][k];
+            double dm, d0, dp;
+            if (G.use_lim) limited_slope_grad<LIMITER>(am, a[0], ap, inv_dx, dm, d0, dp);
+            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
+            double gm = lSx * dm, gp = lSx * dp;
+            lp += lSx * d0;
+            if (cxm) { lp += gm; gm = 0.0; }
+            if (cxp) { lp += gp; gp = 0.0; }
+            sG[0 * B::G_SZ + idx] = gm;
+            sG[1 * B::G_SZ + idx] = gp;
+            if (G.use_lim) limited_slope_grad<LIMITER>(bm, a[0], bp, inv_dx, dm, d0, dp);
+            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
+            gm = lSy * dm;
+            gp = lSy * dp;
+            lp += lSy * d0;
+            if (cym) { lp += gm; gm = 0.0; }
+            if (cyp) { lp += gp; gp = 0.0; }
+            sG[2 * B::G_SZ + idx] = gm;
+            sG[3 * B::G_SZ + idx] = gp;
+            sG[4 * B::G_SZ + idx] = lp;
+        }
+        __syncthreads();
+        // own part plus the neighbours' parts for cells [0, T] (as ar_prims)
+#pragma unroll
+        for (int u = 0; u < B::CPT; ++u) {
+            const int c = tid + u * NT;
+            if (c >= B::C_SZ) break;
+            const int lx = c % B::CW, ly = c / B::CW;
+            int i, j;
+            if (!gcell(lx, ly, i, j)) {
+                lp6[u][f] = 0.0;
+                continue;
+            }
+            const bool hl = i > 0 || px, hr = i < nx - 1 || px;
+            const bool hb = j > 0 || py, ht = j < ny - 1 || py;
+            const int gidx = (ly + 1) * B::GW + (lx + 1);
+            double acc = sG[4 * B::G_SZ + gidx];
+            if (hl) acc += sG[1 * B::G_SZ + gidx - 1];
+            if (hr) acc += sG[0 * B::G_SZ + gidx + 1];
+            if (hb) acc += sG[3 * B::G_SZ + gidx - B::GW];
+            if (ht) acc += sG[2 * B::G_SZ + gidx + B::GW];
+            lp6[u][f] = acc;
+        }
+        __syncthreads();
+    }
+    // reverse of cons_to_prim for cells [0, T]; lam of the conserved input for owned cells
+#pragma unroll
+    for (int u = 0; u < B::CPT; ++u) {
+        const int c = tid + u * NT;
+        if (c >= B::C_SZ) break;
+        const int lx = c % B::CW, ly = c / B::CW;
+        int i, j;
+        if (!gcell(lx, ly, i, j)) {
+            sB[c] = 0.0;
+            sB[B::C_SZ + c] = 0.0;
+            continue;
+        }
+        const size_t k = (size_t)i + (size_t)nx * j;
+        const double Bx = S.P[4][k], By = S.P[5][k];
+        double lU[4] = {0, 0, 0, 0};
+        double lBx = lp6[u][4] + R.lBc[0][k], lBy = lp6[u][5] + R.lBc[1][k];
+        cons_to_prim_vjp(U[0][k], U[1][k], U[2][k], U[3][k], Bx, By, G.gamma, G.vol, lp6[u], lU,
+                         lBx, lBy);
+        if (lx < TX && ly < TY && x0 + lx < nx && y0 + ly < ny) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lin[q][k] += lU[q];
+        }
+        sB[c] = lBx;
+        sB[B::C_SZ + c] = lBy;
+    }
+    __syncthreads();
+    // face -> cell B average (as ar_bavg), owned cells
+    {
+        const int lx = tid % TX, ly = tid / TX;
+        const int i = x0 + lx, j = y0 + ly;
+        if (i < nx && j < ny) {
+            const size_t k = (size_t)i + (size_t)nx * j;
+            const int c = ly * B::CW + lx;
+            const double *lbx = sB, *lby = sB + B::C_SZ;
+            double ax = (i == 0 && !px) ? lbx[c] : 0.5 * lbx[c];
+            if (i + 1 < nx || px) ax += 0.5 * lbx[c + 1];
+            double ay = (j == 0 && !py) ? lby[c] : 0.5 * lby[c];
+            if (j + 1 < ny || py) ay += 0.5 * lby[c + B::CW];
+            lin[4][k] += ax;
+            lin[5][k] += ay;
+        }
+    }
+}
+
 __global__ void add_kernel(double *const *a, const double *const *b, size_t n) {
     CELL_LOOP(k, n) {
         for (int q = 0; q < 6; ++q) a[q][k] += b[q][k];
@@ -384,6 +889,8 @@ struct Ptr6 {
 
 struct chimera_adj {
     AdjGrid G{};
+    bool fused = true;                        // fused reverse (ar_tile_a / ar_tile_b)
+    int rtiles_x = 0, rntiles = 0, rgrid = 0; // their launch shape (RTX x RTY tiles)
     chimera_scheme_t sch{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -448,17 +955,19 @@ int set_dts(chimera_adj *a, const double *dts, int k) {
 
 using AdjStageFn = void (*)(GridView, StageStore, int, int, int, int, int);
 
-AdjStageFn adj_stage_fn(int solver, int limiter) {
-    static const AdjStageFn t[3][2] = {
-        {adj_stage_kernel<RUSANOV, LIM_VAN_LEER>, adj_stage_kernel<RUSANOV, LIM_MC>},
-        {adj_stage_kernel<HLLE, LIM_VAN_LEER>, adj_stage_kernel<HLLE, LIM_MC>},
-        {adj_stage_kernel<HLLD, LIM_VAN_LEER>, adj_stage_kernel<HLLD, LIM_MC>}};
-    return t[solver - 1][limiter - 1];
+AdjStageFn adj_stage_fn(int solver, int limiter, bool store) {
+    static const AdjStageFn t[2][3][2] = {
+        {{adj_stage_kernel<RUSANOV, LIM_VAN_LEER, false>, adj_stage_kernel<RUSANOV, LIM_MC, false>},
+         {adj_stage_kernel<HLLE, LIM_VAN_LEER, false>, adj_stage_kernel<HLLE, LIM_MC, false>},
+         {adj_stage_kernel<HLLD, LIM_VAN_LEER, false>, adj_stage_kernel<HLLD, LIM_MC, false>}},
+        {{adj_stage_kernel<RUSANOV, LIM_VAN_LEER, true>, adj_stage_kernel<RUSANOV, LIM_MC, true>},
+         {adj_stage_kernel<HLLE, LIM_VAN_LEER, true>, adj_stage_kernel<HLLE, LIM_MC, true>},
+         {adj_stage_kernel<HLLD, LIM_VAN_LEER, true>, adj_stage_kernel<HLLD, LIM_MC, true>}}};
+    return t[store ? 1 : 0][solver - 1][limiter - 1];
 }
 
-// One stage through the fused kernel, keeping the intermediates in S[stage - 1].
-int run_forward_stage(chimera_adj *a, int stage, const Ptr6 &in, const Ptr6 &u0, const Ptr6 &out,
-                      const double *dt_dev) {
+GridView grid_view(const chimera_adj *a, const Ptr6 &in, const Ptr6 &u0, const Ptr6 &out,
+                   const double *dt_dev) {
     GridView g{};
     for (int q = 0; q < 6; ++q) {
         g.uin[q] = in.p[q];
@@ -474,10 +983,46 @@ int run_forward_stage(chimera_adj *a, int stage, const Ptr6 &in, const Ptr6 &u0,
     g.dx = a->G.dx;
     g.vol = a->G.vol;
     g.gamma = a->G.gamma;
-    hipLaunchKernelGGL(adj_stage_fn(a->sch.riemann_solver, a->sch.slope_limiter), dim3(a->grid),
-                       dim3(STAGE_THREADS), 0, a->stream, g, a->S[stage - 1], stage, a->G.use_lim,
-                       a->G.mood, a->tiles_x, a->ntiles);
+    return g;
+}
+
+// One stage through the fused kernel; with store, keeping the intermediates in
+// S[stage - 1].
+int run_forward_stage(chimera_adj *a, int stage, const Ptr6 &in, const Ptr6 &u0, const Ptr6 &out,
+                      const double *dt_dev, bool store = true) {
+    GridView g = grid_view(a, in, u0, out, dt_dev);
+    hipLaunchKernelGGL(adj_stage_fn(a->sch.riemann_solver, a->sch.slope_limiter, store),
+                       dim3(a->grid), dim3(STAGE_THREADS), 0, a->stream, g, a->S[stage - 1], stage,
+                       a->G.use_lim, a->G.mood, a->tiles_x, a->ntiles);
     return launch_check("adjoint forward stage");
+}
+
+using TileAFn = void (*)(GridView, AdjGrid, StageBufs, RevBufs, const double *const *,
+                         double *const *, double *const *, int, double, int, int);
+using TileBFn = void (*)(AdjGrid, const double *const *, StageStore, RevBufs, double *const *, int,
+                         int);
+
+TileAFn tile_a_fn(int solver, int limiter) {
+    static const TileAFn t[3][2] = {
+        {ar_tile_a<RUSANOV, LIM_VAN_LEER, RTX, RTY>, ar_tile_a<RUSANOV, LIM_MC, RTX, RTY>},
+        {ar_tile_a<HLLE, LIM_VAN_LEER, RTX, RTY>, ar_tile_a<HLLE, LIM_MC, RTX, RTY>},
+        {ar_tile_a<HLLD, LIM_VAN_LEER, RTX, RTY>, ar_tile_a<HLLD, LIM_MC, RTX, RTY>}};
+    return t[solver - 1][limiter - 1];
+}
+
+// The reverse of one stage through the fused kernels: in is the stage input (host
+// pointer set and device table tin), u0 the step input.
+int run_reverse_stage_fused(chimera_adj *a, int stage, const Ptr6 &in, Tab tin, const Ptr6 &u0,
+                            Tab tlout, Tab tlin, double dt) {
+    GridView g = grid_view(a, in, u0, in, nullptr);
+    hipLaunchKernelGGL(tile_a_fn(a->sch.riemann_solver, a->sch.slope_limiter), dim3(a->rgrid),
+                       dim3(RTX * RTY), 0, a->stream, g, a->G, a->S[stage - 1], a->R, tlout, tlin,
+                       table(a, TB_LU0), stage, dt, a->rtiles_x, a->rntiles);
+    TileBFn fb = a->sch.slope_limiter == LIM_MC ? ar_tile_b<LIM_MC, RTX, RTY>
+                                                : ar_tile_b<LIM_VAN_LEER, RTX, RTY>;
+    hipLaunchKernelGGL(fb, dim3(a->rgrid), dim3(RTX * RTY), 0, a->stream, a->G, tin, a->S[stage - 1],
+                       a->R, tlin, a->rtiles_x, a->rntiles);
+    return launch_check("adjoint fused reverse stage");
 }
 
 int run_reverse_stage(chimera_adj *a, int stage, Tab tin, Tab tu0, Tab tlout, Tab tlin, double dt) {
@@ -507,6 +1052,21 @@ int run_reverse_stage(chimera_adj *a, int stage, Tab tin, Tab tu0, Tab tlout, Ta
 int vjp_core(chimera_adj *a, const Ptr6 &hs, Tab ts, const double *dt_dev, double dt, Tab tlout,
              Tab tlin) {
     const int nb = nblocks(a->n);
+    if (a->fused) {
+        // forward stages 1 and 2 for the stage inputs U^(1), U^(2); the reverse kernels
+        // recompute everything else
+        CH_TRY(run_forward_stage(a, 1, hs, hs, a->stage_state[1], dt_dev, false));
+        CH_TRY(run_forward_stage(a, 2, a->stage_state[1], hs, a->stage_state[2], dt_dev, false));
+        hipLaunchKernelGGL(zero6_kernel, dim3(nb), dim3(BS), 0, a->stream, table(a, TB_LU0), a->n);
+        CH_TRY(run_reverse_stage_fused(a, 3, a->stage_state[2], table(a, TB_S2), hs, tlout,
+                                       table(a, TB_LB), dt));
+        CH_TRY(run_reverse_stage_fused(a, 2, a->stage_state[1], table(a, TB_S1), hs,
+                                       table(a, TB_LB), table(a, TB_LA), dt));
+        CH_TRY(run_reverse_stage_fused(a, 1, hs, ts, hs, table(a, TB_LA), tlin, dt));
+        hipLaunchKernelGGL(add_kernel, dim3(nb), dim3(BS), 0, a->stream, tlin,
+                           (const double *const *)table(a, TB_LU0), a->n);
+        return launch_check("adjoint accumulate");
+    }
     // forward: keep every stage's intermediates
     CH_TRY(run_forward_stage(a, 1, hs, hs, a->stage_state[1], dt_dev));
     CH_TRY(run_forward_stage(a, 2, a->stage_state[1], hs, a->stage_state[2], dt_dev));
@@ -580,6 +1140,9 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
     a->tiles_x = (nx + ATX - 1) / ATX;
     a->ntiles = a->tiles_x * ((ny + ATY - 1) / ATY);
     a->grid = (a->ntiles + 7) / 8 * 8;
+    a->rtiles_x = (nx + RTX - 1) / RTX;
+    a->rntiles = a->rtiles_x * ((ny + RTY - 1) / RTY);
+    a->rgrid = (a->rntiles + 7) / 8 * 8;
     if (stream) {
         a->stream = (hipStream_t)stream;
     } else {
@@ -732,3 +1295,13 @@ extern "C" int chimera_adj_segment_vjp(chimera_adj_t *a, const double *const *st
 }
 
 extern "C" void *chimera_adj_stream(chimera_adj_t *a) { return a ? (void *)a->stream : nullptr; }
+
+// 1 (default): the fused reverse kernels; 0: the stored-intermediate kernels.  Both
+// give the same bits; the second is kept as the cross-check of the first.
+extern "C" int chimera_adj_set_fused(chimera_adj_t *a, int fused) {
+    CH_REQUIRE(a, "NULL adjoint");
+    CH_REQUIRE(fused == 0 || fused == 1, "fused must be 0 or 1, got %d", fused);
+    CH_HIP(hipStreamSynchronize(a->stream));
+    a->fused = fused != 0;
+    return CHIMERA_OK;
+}

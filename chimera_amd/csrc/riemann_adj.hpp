@@ -322,14 +322,20 @@ __device__ __forceinline__ void rusanov_vjp(const FaceState &L, const FaceState 
     side_energy_rev(R, eR, g, lenR, lpthR, lhR, lamR);
 }
 
+// Directions per forward-mode pass of the HLLD face VJP (12 inputs: 12 / N passes, each
+// carrying the primal and N tangents).  Every N gives the same bits.
+#ifndef CHIMERA_HLLD_DUAL_N
+#define CHIMERA_HLLD_DUAL_N 1
+#endif
+
 // Face VJP of the solver SOLVER: reverse mode for Rusanov and HLLE, forward-mode dual
-// numbers (one pass per input) for HLLD.
+// numbers for HLLD.
 template <int SOLVER>
 __device__ __forceinline__ void riemann_rev(const FaceState &L, const FaceState &R, double g,
                                             const double lamF[5], double lamL[6], double lamR[6]) {
     if (SOLVER == RUSANOV) rusanov_vjp(L, R, g, lamF, lamL, lamR);
     else if (SOLVER == HLLE) hlle_vjp(L, R, g, lamF, lamL, lamR);
-    else riemann_vjp<HLLD, 1>(L, R, g, lamF, lamL, lamR);
+    else riemann_vjp<HLLD, CHIMERA_HLLD_DUAL_N>(L, R, g, lamF, lamL, lamR);
 }
 
 }  // namespace chimera

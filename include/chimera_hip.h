@@ -275,6 +275,12 @@ int chimera_adj_segment_vjp(chimera_adj_t *adj, const double *const *state0, con
                             int nsteps, double *const *lam);
 void *chimera_adj_stream(chimera_adj_t *adj);
 
+/* Which reverse kernels the adjoint runs: 1 (default) the fused LDS-tiled pair that
+ * recompute the forward stage in place of reading stored intermediates, 0 the
+ * stored-intermediate kernels.  Both give the same bits; 0 is the cross-check.  No
+ * counterpart in the reference (it has no derivatives). */
+int chimera_adj_set_fused(chimera_adj_t *adj, int fused);
+
 #ifdef __cplusplus
 }
 #endif
