@@ -100,7 +100,6 @@ def _declare(L: C.CDLL) -> None:
         "chimera_ctx_upload": [vp, C.POINTER(vp)],
         "chimera_ctx_download": [vp, C.POINTER(vp)],
         "chimera_ctx_steps": [vp, i, i, c_dp, i],
-        "chimera_ctx_steps_record": [vp, i, c_dp, C.POINTER(vp)],
         "chimera_ctx_get_time": [vp, C.POINTER(d), C.POINTER(i)],
         "chimera_ctx_set_time": [vp, d, i],
         "chimera_ctx_last_dt": [vp, C.POINTER(d)],

@@ -906,8 +906,7 @@ struct chimera_adj {
     // dt of the steps being processed (device; the stage kernel reads *dt)
     double *dt_dev = nullptr;
     int dt_cap = 0;
-    // checkpoint recomputation: fused forward and the recomputed states
-    chimera_ctx_t *fwd = nullptr;
+    // checkpoint recomputation: the recomputed stage inputs
     double *traj = nullptr;       // traj_cap x 6 fields
     double **traj_tab = nullptr;  // device tables, 6 pointers per slot
     int traj_cap = 0;
@@ -1049,19 +1048,26 @@ int run_reverse_stage(chimera_adj *a, int stage, Tab tin, Tab tu0, Tab tlout, Ta
 // lam_in = (dU^{n+1}/dU^n)^T lam_out for one step at U^n (host pointers hs, device
 // table ts), dt = *dt_dev; asynchronous.  tlout and tlin must differ; neither may be
 // LA, LB or LU0.
+// The stage inputs U^(1), U^(2) of the step may be supplied (host pointer sets s1, s2
+// and device tables t1, t2, e.g. recorded by the segment recompute); otherwise the
+// fused path computes them with two forward stages.
 int vjp_core(chimera_adj *a, const Ptr6 &hs, Tab ts, const double *dt_dev, double dt, Tab tlout,
-             Tab tlin) {
+             Tab tlin, const Ptr6 *s1 = nullptr, Tab t1 = nullptr, const Ptr6 *s2 = nullptr,
+             Tab t2 = nullptr) {
     const int nb = nblocks(a->n);
     if (a->fused) {
-        // forward stages 1 and 2 for the stage inputs U^(1), U^(2); the reverse kernels
-        // recompute everything else
-        CH_TRY(run_forward_stage(a, 1, hs, hs, a->stage_state[1], dt_dev, false));
-        CH_TRY(run_forward_stage(a, 2, a->stage_state[1], hs, a->stage_state[2], dt_dev, false));
+        // the reverse kernels recompute everything else from the stage inputs
+        if (!s1 || !s2) {
+            CH_TRY(run_forward_stage(a, 1, hs, hs, a->stage_state[1], dt_dev, false));
+            CH_TRY(run_forward_stage(a, 2, a->stage_state[1], hs, a->stage_state[2], dt_dev, false));
+            s1 = &a->stage_state[1];
+            s2 = &a->stage_state[2];
+            t1 = table(a, TB_S1);
+            t2 = table(a, TB_S2);
+        }
         hipLaunchKernelGGL(zero6_kernel, dim3(nb), dim3(BS), 0, a->stream, table(a, TB_LU0), a->n);
-        CH_TRY(run_reverse_stage_fused(a, 3, a->stage_state[2], table(a, TB_S2), hs, tlout,
-                                       table(a, TB_LB), dt));
-        CH_TRY(run_reverse_stage_fused(a, 2, a->stage_state[1], table(a, TB_S1), hs,
-                                       table(a, TB_LB), table(a, TB_LA), dt));
+        CH_TRY(run_reverse_stage_fused(a, 3, *s2, t2, hs, tlout, table(a, TB_LB), dt));
+        CH_TRY(run_reverse_stage_fused(a, 2, *s1, t1, hs, table(a, TB_LB), table(a, TB_LA), dt));
         CH_TRY(run_reverse_stage_fused(a, 1, hs, ts, hs, table(a, TB_LA), tlin, dt));
         hipLaunchKernelGGL(add_kernel, dim3(nb), dim3(BS), 0, a->stream, tlin,
                            (const double *const *)table(a, TB_LU0), a->n);
@@ -1112,7 +1118,6 @@ Ptr6 traj_slot(chimera_adj *a, int s) {
 extern "C" int chimera_adj_destroy(chimera_adj_t *a) {
     if (!a) return CHIMERA_OK;
     if (a->stream) (void)hipStreamSynchronize(a->stream);
-    if (a->fwd) chimera_ctx_destroy(a->fwd);
     (void)hipFree(a->arena);
     (void)hipFree(a->dptr);
     (void)hipFree(a->dt_dev);
@@ -1202,8 +1207,7 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
     for (auto &x : a->lam_c.p) x = take();
     for (auto &x : a->lam_u0.p) x = take();
     for (auto &x : a->scratch_out.p) x = take();
-    int rc = chimera_ctx_create(&a->fwd, nx, ny, dx, gamma, bc, scheme, a->stream);
-    if (rc == CHIMERA_OK) rc = set_table(a, TB_S0, a->stage_state[0].p);
+    int rc = set_table(a, TB_S0, a->stage_state[0].p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_S1, a->stage_state[1].p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_S2, a->stage_state[2].p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_LA, a->lam_a.p);
@@ -1263,24 +1267,32 @@ extern "C" int chimera_adj_segment_vjp(chimera_adj_t *a, const double *const *st
     CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
     for (int s = 0; s < nsteps; ++s)
         CH_REQUIRE(dt_in[s] > 0.0, "dt_in[%d] = %g is not positive", s, dt_in[s]);
-    // slots 1..nsteps-1 hold U^1..U^{nsteps-1}; U^0 is the caller's state0
-    CH_TRY(ensure_traj(a, nsteps));
+    CH_TRY(ensure_traj(a, 3 * nsteps));
     CH_TRY(set_dts(a, dt_in, nsteps));
     CH_TRY(set_table(a, TB_IN, state0));
     CH_TRY(set_table(a, TB_LIN, (const double *const *)lam));
-    if (nsteps > 1) {
-        CH_TRY(chimera_ctx_upload(a->fwd, state0));
-        std::vector<double *> traj((size_t)(nsteps - 1) * 6);
-        for (size_t k = 0; k < traj.size(); ++k) traj[k] = a->traj + (6 + k) * a->n;
-        CH_TRY(chimera_ctx_steps_record(a->fwd, nsteps - 1, dt_in, traj.data()));
-    }
-    // lam ping-pongs between the caller's arrays and lam_c
+    // recompute the segment, keeping every stage input: slot 3s holds U^s (s >= 1;
+    // U^0 is the caller's state0), slots 3s+1 and 3s+2 the stage outputs U^(1), U^(2)
     const Ptr6 h0 = ptr6(state0);
+    for (int s = 0; s < nsteps; ++s) {
+        const Ptr6 in = s == 0 ? h0 : traj_slot(a, 3 * s);
+        const double *dts = a->dt_dev + s;
+        CH_TRY(run_forward_stage(a, 1, in, in, traj_slot(a, 3 * s + 1), dts, false));
+        CH_TRY(run_forward_stage(a, 2, traj_slot(a, 3 * s + 1), in, traj_slot(a, 3 * s + 2), dts,
+                                 false));
+        if (s + 1 < nsteps)
+            CH_TRY(run_forward_stage(a, 3, traj_slot(a, 3 * s + 2), in, traj_slot(a, 3 * s + 3), dts,
+                                     false));
+    }
+    auto slot_tab = [&](int k) { return (Tab)(a->traj_tab + 6 * (size_t)k); };
+    // lam ping-pongs between the caller's arrays and lam_c
     Tab cur = table(a, TB_LIN), other = table(a, TB_LC);
     for (int s = nsteps - 1; s >= 0; --s) {
-        const Ptr6 hs = s == 0 ? h0 : traj_slot(a, s);
-        Tab ts = s == 0 ? table(a, TB_IN) : (Tab)(a->traj_tab + 6 * (size_t)s);
-        CH_TRY(vjp_core(a, hs, ts, a->dt_dev + s, dt_in[s], cur, other));
+        const Ptr6 hs = s == 0 ? h0 : traj_slot(a, 3 * s);
+        Tab ts = s == 0 ? table(a, TB_IN) : slot_tab(3 * s);
+        const Ptr6 s1 = traj_slot(a, 3 * s + 1), s2 = traj_slot(a, 3 * s + 2);
+        CH_TRY(vjp_core(a, hs, ts, a->dt_dev + s, dt_in[s], cur, other, &s1, slot_tab(3 * s + 1),
+                        &s2, slot_tab(3 * s + 2)));
         Tab t = cur;
         cur = other;
         other = t;

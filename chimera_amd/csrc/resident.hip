@@ -541,38 +541,6 @@ extern "C" int chimera_ctx_steps(chimera_ctx_t *c, int nsteps, int dt_mode, cons
     return CHIMERA_OK;
 }
 
-namespace chimera {
-struct Fields6 {
-    double *p[6];
-};
-__global__ void copy6_kernel(Fields6 src, Fields6 dst, size_t n) {
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (size_t)gridDim.x * blockDim.x)
-        for (int q = 0; q < 6; ++q) dst.p[q][k] = src.p[q][k];
-}
-}  // namespace chimera
-
-extern "C" int chimera_ctx_steps_record(chimera_ctx_t *c, int nsteps, const double *dt_in,
-                                        double *const *traj) {
-    CH_REQUIRE(c && traj, "NULL argument");
-    CH_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
-    if (nsteps == 0) return CHIMERA_OK;
-    CH_TRY(prepare_prescribed(c, nsteps, dt_in));
-    const size_t n = (size_t)c->nx * c->ny;
-    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
-    for (int s = 0; s < nsteps; ++s) {
-        CH_TRY(launch_one_step(c, CHIMERA_DT_PRESCRIBED, true, false));
-        Fields6 src, dst;
-        for (int q = 0; q < 6; ++q) {
-            src.p[q] = c->S[c->cur][q];
-            dst.p[q] = traj[6 * (size_t)s + q];
-        }
-        hipLaunchKernelGGL(copy6_kernel, dim3(blocks), dim3(256), 0, c->stream, src, dst, n);
-        CH_HIP(hipGetLastError());
-    }
-    return CHIMERA_OK;
-}
-
 extern "C" int chimera_ctx_sync(chimera_ctx_t *c) {
     CH_REQUIRE(c, "ctx is NULL");
     CH_HIP(hipStreamSynchronize(c->stream));

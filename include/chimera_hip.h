@@ -180,11 +180,6 @@ int chimera_ctx_download(chimera_ctx_t *ctx, double *const *state);
  * CHIMERA_DT_PRESCRIBED, else NULL.  use_graph: replay a captured hipGraph per step. */
 int chimera_ctx_steps(chimera_ctx_t *ctx, int nsteps, int dt_mode, const double *dt_in,
                       int use_graph);
-/* nsteps steps with the prescribed dts dt_in (host memory), copying the state after
- * step s into the six device arrays traj[6*s .. 6*s+5] (checkpoint recomputation).
- * No counterpart in the reference. */
-int chimera_ctx_steps_record(chimera_ctx_t *ctx, int nsteps, const double *dt_in,
-                             double *const *traj);
 /* Simulation time and the reference's outputCount (main.f90:61-62,279-292). */
 int chimera_ctx_get_time(chimera_ctx_t *ctx, double *t, int *out_count);
 int chimera_ctx_set_time(chimera_ctx_t *ctx, double t, int out_count);
@@ -268,9 +263,10 @@ int chimera_adj_forward_step(chimera_adj_t *adj, const double *const *state_in, 
 /* lam_in = (dU^{n+1}/dU^n)^T lam_out at U^n = state_n with dt frozen. */
 int chimera_adj_step_vjp(chimera_adj_t *adj, const double *const *state_n, double dt,
                          const double *const *lam_out, double *const *lam_in);
-/* Reverse sweep over one checkpoint segment of nsteps steps: recompute U^1..U^{nsteps-1}
- * from state0 = U^0 with the fused forward and the dts dt_in[0..nsteps-1] (host memory),
- * then lam <- (dU^{nsteps}/dU^0)^T lam in place (device arrays). */
+/* Reverse sweep over one checkpoint segment of nsteps steps: recompute the segment from
+ * state0 = U^0 with the fused forward stage kernel and the dts dt_in[0..nsteps-1] (host
+ * memory), keeping every stage input (3 * nsteps states of device memory, allocated on
+ * first use and kept), then lam <- (dU^{nsteps}/dU^0)^T lam in place (device arrays). */
 int chimera_adj_segment_vjp(chimera_adj_t *adj, const double *const *state0, const double *dt_in,
                             int nsteps, double *const *lam);
 void *chimera_adj_stream(chimera_adj_t *adj);
