@@ -322,20 +322,521 @@ __device__ __forceinline__ void rusanov_vjp(const FaceState &L, const FaceState 
     side_energy_rev(R, eR, g, lenR, lpthR, lhR, lamR);
 }
 
-// Directions per forward-mode pass of the HLLD face VJP (12 inputs: 12 / N passes, each
-// carrying the primal and N tangents).  Every N gives the same bits.
-#ifndef CHIMERA_HLLD_DUAL_N
-#define CHIMERA_HLLD_DUAL_N 1
-#endif
+// ---- HLLD (src/mhd_flux.f90:490-777) ------------------------------------------------
+//
+// The forward recompute keeps every intermediate and branch of hlld() (mhd_phys.hpp);
+// the reverse sweep then runs through the branch that produced the flux: c_f cap ->
+// upwind side, supersonic upwind, fan-edge catch and the non-finite / out-of-bound
+// catch-all -> HLL average, else the star / double-star states with the p_T* floor,
+// the degenerate-denominator and |B_n| -> 0 branches.  Conventions match the
+// dual-number solver: fmax / fmin take the argument the FP64 pass returned (the first
+// on ties), sqrt and |x| have derivative 0 at 0, the SM denominator floor is a
+// constant.
 
-// Face VJP of the solver SOLVER: reverse mode for Rusanov and HLLE, forward-mode dual
-// numbers for HLLD.
+struct HlldSide {
+    double r, u, w, pT, B, C;   // rho, u_n, u_t, total pressure, B_n, B_t
+    SideRev e;                  // halfB2, thermal pressure (floor), energy
+    double vdotB, B2, ca, cs2, b2, tmp, s1, cf;
+    bool disc_floor;
+};
+
+__device__ __forceinline__ HlldSide hlld_side(const FaceState &s, double g) {
+    HlldSide h;
+    h.r = s.rho;
+    h.u = s.u;
+    h.w = s.w;
+    h.pT = s.P;
+    h.B = s.Bn;
+    h.C = s.Bt;
+    h.e = side_energy(s, g);
+    h.vdotB = h.u * h.B + h.w * h.C;
+    h.B2 = h.B * h.B;
+    h.ca = h.B2 / h.r;
+    h.cs2 = g * h.e.pth / h.r;
+    h.b2 = (h.B2 + h.C * h.C) / h.r;
+    h.tmp = h.cs2 + h.b2;
+    const double d = h.tmp * h.tmp - 4.0 * h.cs2 * h.ca;
+    h.disc_floor = 0.0 > d;
+    h.s1 = sqrt(h.disc_floor ? 0.0 : d);
+    h.cf = sqrt(0.5 * (h.tmp + h.s1));
+    return h;
+}
+
+// physical flux of one side in the HLLD form
+__device__ __forceinline__ void hlld_phys(const HlldSide &h, double F[5]) {
+    F[0] = h.r * h.u;
+    F[1] = h.r * (h.u * h.u) + h.pT - h.B * h.B;
+    F[2] = h.r * h.u * h.w - h.B * h.C;
+    F[3] = (h.e.en + h.pT) * h.u - h.B * h.vdotB;
+    F[4] = h.C * h.u - h.B * h.w;
+}
+
+// adjoints gathered per side before they reach its six inputs
+struct HlldAdj {
+    double l[6];
+    double len, lpth, lhalf, lB2, lvdotB, lF[5];
+};
+
+__device__ __forceinline__ void hlld_adj_zero(HlldAdj &a) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.l[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) a.lF[q] = 0.0;
+    a.len = a.lpth = a.lhalf = a.lB2 = a.lvdotB = 0.0;
+}
+
+// c_f = sqrt(0.5 (tmp + sqrt(max(tmp^2 - 4 cs2 ca, 0)))) and its inputs
+__device__ __forceinline__ void hlld_cf_rev(const HlldSide &h, double g, double lcf, HlldAdj &a) {
+    const double lx = h.cf > 0.0 ? lcf * (0.5 / h.cf) : 0.0;
+    double ltmp = 0.5 * lx;
+    const double ls1 = 0.5 * lx;
+    const double ld = (!h.disc_floor && h.s1 > 0.0) ? ls1 * (0.5 / h.s1) : 0.0;
+    ltmp += ld * (2.0 * h.tmp);
+    double lcs2 = -ld * (4.0 * h.ca);
+    const double lca = -ld * (4.0 * h.cs2);
+    lcs2 += ltmp;
+    const double lb2 = ltmp;
+    a.lB2 += lb2 / h.r;
+    a.l[5] += lb2 * (2.0 * h.C) / h.r;
+    a.l[0] -= lb2 * h.b2 / h.r;
+    a.lpth += lcs2 * g / h.r;
+    a.l[0] -= lcs2 * h.cs2 / h.r;
+    a.lB2 += lca / h.r;
+    a.l[0] -= lca * h.ca / h.r;
+}
+
+// phys flux, v.B, B^2 and energy of one side -> its inputs (written into lam)
+__device__ __forceinline__ void hlld_side_finish(const FaceState &s, const HlldSide &h, double g,
+                                                 HlldAdj &a, double lam[6]) {
+    phys_flux_rev(s, h.e.en, a.lF, a.l, a.len);
+    a.l[1] += a.lvdotB * h.B;
+    a.l[4] += a.lvdotB * h.u;
+    a.l[2] += a.lvdotB * h.C;
+    a.l[5] += a.lvdotB * h.w;
+    a.l[4] += a.lB2 * (2.0 * h.B);
+    side_energy_rev(s, h.e, g, a.len, a.lpth, a.lhalf, a.l);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) lam[q] = a.l[q];
+}
+
+// one side's star state (Miyoshi-Kusano): forward values
+struct HlldStar {
+    double a, dS, rs, k, den, t, nv, vs, Bs, vdotBs, ens, sq;
+    bool cond, sq_floor;
+};
+
+__device__ __forceinline__ HlldStar hlld_star(const HlldSide &h, double S, double SM, double pTs) {
+    HlldStar st;
+    st.a = h.r * (S - h.u);
+    st.dS = S - SM;
+    st.rs = st.a / st.dS;
+    st.k = st.a * st.dS;
+    st.den = st.k - h.B2;
+    st.cond = fabs(st.den) > 1.0e-3 * fmax(fmax(fabs(st.k), h.B2), 1.0e-30);
+    st.vs = h.w;
+    st.Bs = h.C;
+    st.t = S - h.u;
+    st.nv = 0.0;
+    if (st.cond) {
+        st.nv = h.B * h.C * (SM - h.u);
+        st.vs = h.w - st.nv / st.den;
+        st.Bs = h.C * (h.r * (st.t * st.t) - h.B2) / st.den;
+    }
+    st.vdotBs = SM * h.B + st.vs * st.Bs;
+    st.ens = (h.e.en * (S - h.u) - h.pT * h.u + pTs * SM + h.B * (h.vdotB - st.vdotBs)) / st.dS;
+    st.sq_floor = RHO_FLOOR > st.rs;
+    st.sq = sqrt(st.sq_floor ? RHO_FLOOR : st.rs);
+    return st;
+}
+
+// reverse of hlld_star given the adjoints of rs, vs, Bs, vdotBs, ens and sq; adds to the
+// side's adjoints and to those of S, SM, p_T* and of a (returned through la)
+__device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar &st, double S,
+                                              double SM, double pTs, double lrs, double lvs,
+                                              double lBs, double lvdotBs, double lens, double lsq,
+                                              HlldAdj &a, double &lS, double &lSM, double &lpTs,
+                                              double &la) {
+    if (!st.sq_floor && st.sq > 0.0) lrs += lsq * (0.5 / st.sq);
+    double ldS = 0.0;
+    // ens = numE / dS
+    const double lnumE = lens / st.dS;
+    ldS -= lens * st.ens / st.dS;
+    a.len += lnumE * (S - h.u);
+    lS += lnumE * h.e.en;
+    a.l[1] -= lnumE * h.e.en;
+    a.l[3] -= lnumE * h.u;
+    a.l[1] -= lnumE * h.pT;
+    lpTs += lnumE * SM;
+    lSM += lnumE * pTs;
+    a.l[4] += lnumE * (h.vdotB - st.vdotBs);
+    a.lvdotB += lnumE * h.B;
+    lvdotBs -= lnumE * h.B;
+    // vdotBs = SM B + vs Bs
+    lSM += lvdotBs * h.B;
+    a.l[4] += lvdotBs * SM;
+    lvs += lvdotBs * st.Bs;
+    lBs += lvdotBs * st.vs;
+    double lk = 0.0;
+    if (st.cond) {
+        double lden = 0.0;
+        // Bs = C (r t^2 - B^2) / den
+        const double lnb = lBs / st.den;
+        lden -= lBs * st.Bs / st.den;
+        a.l[5] += lnb * (h.r * (st.t * st.t) - h.B2);
+        const double lin = lnb * h.C;
+        a.l[0] += lin * (st.t * st.t);
+        const double lt = lin * h.r * (2.0 * st.t);
+        a.lB2 -= lin;
+        lS += lt;
+        a.l[1] -= lt;
+        // vs = w - nv / den, nv = B C (SM - u)
+        a.l[2] += lvs;
+        const double lnv = -lvs / st.den;
+        lden += lvs * (st.nv / st.den) / st.den;
+        a.l[4] += lnv * h.C * (SM - h.u);
+        a.l[5] += lnv * h.B * (SM - h.u);
+        lSM += lnv * (h.B * h.C);
+        a.l[1] -= lnv * (h.B * h.C);
+        // den = k - B^2
+        lk += lden;
+        a.lB2 -= lden;
+    } else {
+        a.l[2] += lvs;
+        a.l[5] += lBs;
+    }
+    // k = a dS, rs = a / dS, dS = S - SM
+    la += lk * st.dS;
+    ldS += lk * st.a;
+    la += lrs / st.dS;
+    ldS -= lrs * st.rs / st.dS;
+    lS += ldS;
+    lSM -= ldS;
+}
+
+// HLL average of the two physical fluxes (no span floor) -> adjoints of FL, FR, SL,
+// SR and of the conserved jumps
+__device__ __forceinline__ void hll_avg_rev(const HlldSide &hL, const HlldSide &hR,
+                                            const double FL[5], const double FR[5], double SL,
+                                            double SR, const double lamF[5], HlldAdj &aL,
+                                            HlldAdj &aR, double &lSL, double &lSR) {
+    const double dU[5] = {hR.r - hL.r, hR.r * hR.u - hL.r * hL.u, hR.r * hR.w - hL.r * hL.w,
+                          hR.e.en - hL.e.en, hR.C - hL.C};
+    const double inv = 1.0 / (SR - SL);
+    const double sLR = SL * SR;
+    double linv = 0.0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const double num = SR * FL[c] - SL * FR[c] + sLR * dU[c];
+        const double lc = lamF[c];
+        linv += lc * num;
+        aL.lF[c] += lc * SR * inv;
+        aR.lF[c] -= lc * SL * inv;
+        const double ld = lc * sLR * inv;
+        lSR += lc * inv * (FL[c] + SL * dU[c]);
+        lSL += lc * inv * (SR * dU[c] - FR[c]);
+        if (c == 0) {
+            aR.l[0] += ld;
+            aL.l[0] -= ld;
+        } else if (c == 1) {
+            aR.l[0] += ld * hR.u;
+            aR.l[1] += ld * hR.r;
+            aL.l[0] -= ld * hL.u;
+            aL.l[1] -= ld * hL.r;
+        } else if (c == 2) {
+            aR.l[0] += ld * hR.w;
+            aR.l[2] += ld * hR.r;
+            aL.l[0] -= ld * hL.w;
+            aL.l[2] -= ld * hL.r;
+        } else if (c == 3) {
+            aR.len += ld;
+            aL.len -= ld;
+        } else {
+            aR.l[5] += ld;
+            aL.l[5] -= ld;
+        }
+    }
+    const double lspan = -linv * inv * inv;
+    lSR += lspan;
+    lSL -= lspan;
+}
+
+__device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R, double g,
+                                         const double lamF[5], double lamL[6], double lamR[6]) {
+    const double EPS_FAN = 1.0e-6, cf_max = 50.0;
+    const HlldSide hL = hlld_side(L, g), hR = hlld_side(R, g);
+    double FL[5], FR[5];
+    hlld_phys(hL, FL);
+    hlld_phys(hR, FR);
+    HlldAdj aL, aR;
+    hlld_adj_zero(aL);
+    hlld_adj_zero(aR);
+
+    const bool capL = cf_max < hL.cf, capR = cf_max < hR.cf;
+    const double cfL = capL ? cf_max : hL.cf, cfR = capR ? cf_max : hR.cf;
+    const double xa = hL.u - cfL, xb = hR.u - cfR;
+    const bool sl_right = xb < xa;
+    const double SL = sl_right ? xb : xa;
+    const double ya = hL.u + cfL, yb = hR.u + cfR;
+    const bool sr_right = yb > ya;
+    const double SR = sr_right ? yb : ya;
+
+    double lSL = 0.0, lSR = 0.0;
+    bool speeds = true;   // SL / SR (hence c_f) enter the flux
+    if (cfL >= cf_max || cfR >= cf_max) {
+        HlldAdj &a = (cfL <= cfR) ? aL : aR;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) a.lF[c] += lamF[c];
+        speeds = false;
+    } else if (SL >= 0.0) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) aL.lF[c] += lamF[c];
+        speeds = false;
+    } else if (SR <= 0.0) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) aR.lF[c] += lamF[c];
+        speeds = false;
+    } else {
+        const double aLv = hL.r * (SL - hL.u), aRv = hR.r * (SR - hR.u);
+        double denMD = aRv - aLv;
+        const bool md_floor = fabs(denMD) < 1.0e-15;
+        if (md_floor) denMD = copysign(1.0e-15, denMD);
+        const double num = hR.r * hR.u * (SR - hR.u) - hL.r * hL.u * (SL - hL.u) - (hR.pT - hL.pT);
+        const double SM = num / denMD;
+        const double pTs0 = hL.pT + aLv * (SM - hL.u);
+        bool hll = SM <= SL + EPS_FAN * (SR - SL) || SM >= SR - EPS_FAN * (SR - SL);
+        if (!hll) {
+            const bool pts_floor = P_FLOOR > pTs0;
+            const double pTs = pts_floor ? P_FLOOR : pTs0;
+            const HlldStar sL = hlld_star(hL, SL, SM, pTs), sR = hlld_star(hR, SR, SM, pTs);
+            const double Bavg = 0.5 * (hL.B + hR.B);
+            const double absB = fabs(Bavg);
+            const bool big = absB > 1.0e-10;
+            double SLs = SM - absB / sL.sq, SRs = SM + absB / sR.sq;
+            double vss, Bss, enLss, enRss, vdotBss = 0.0, sgn = 1.0, dss = 1.0;
+            if (big) {
+                sgn = copysign(1.0, Bavg);
+                dss = sL.sq + sR.sq;
+                vss = (sL.sq * sL.vs + sR.sq * sR.vs + sgn * (sR.Bs - sL.Bs)) / dss;
+                Bss = (sL.sq * sR.Bs + sR.sq * sL.Bs + sgn * sL.sq * sR.sq * (sR.vs - sL.vs)) / dss;
+                vdotBss = SM * Bavg + vss * Bss;
+                enLss = sL.ens - sL.sq * sgn * (sL.vdotBs - vdotBss);
+                enRss = sR.ens + sR.sq * sgn * (sR.vdotBs - vdotBss);
+            } else {
+                vss = 0.5 * (sL.vs + sR.vs);
+                Bss = 0.5 * (sL.Bs + sR.Bs);
+                enLss = sL.ens;
+                enRss = sR.ens;
+                SLs = SM;
+                SRs = SM;
+            }
+            // the forward flux, for the catch-all check
+            const bool left = SM >= 0.0;
+            const HlldSide &hX = left ? hL : hR;
+            const HlldStar &sX = left ? sL : sR;
+            const double S = left ? SL : SR, Ss = left ? SLs : SRs;
+            const double enXss = left ? enLss : enRss;
+            const double *FX = left ? FL : FR;
+            const bool star2 = left ? !(SLs >= 0.0) : !(SRs <= 0.0);
+            double f[5];
+            f[0] = FX[0] + S * (sX.rs - hX.r);
+            f[1] = FX[1] + S * (sX.rs * SM - hX.r * hX.u);
+            f[2] = FX[2] + S * (sX.rs * sX.vs - hX.r * hX.w);
+            f[3] = FX[3] + S * (sX.ens - hX.e.en);
+            f[4] = FX[4] + S * (sX.Bs - hX.C);
+            if (star2) {
+                f[2] = f[2] + Ss * sX.rs * (vss - sX.vs);
+                f[3] = f[3] + Ss * (enXss - sX.ens);
+                f[4] = f[4] + Ss * (Bss - sX.Bs);
+            }
+            const double Smax = fmax(fabs(SL), fabs(SR));
+            const bool ok =
+                finite_in(f[0], fmax(fabs(FL[0]), fabs(FR[0])) + Smax * (hL.r + hR.r)) &&
+                finite_in(f[1], fmax(fabs(FL[1]), fabs(FR[1])) +
+                                    Smax * (fabs(hL.r * hL.u) + fabs(hR.r * hR.u))) &&
+                finite_in(f[2], fmax(fabs(FL[2]), fabs(FR[2])) +
+                                    Smax * (fabs(hL.r * hL.w) + fabs(hR.r * hR.w))) &&
+                finite_in(f[3], fmax(fabs(FL[3]), fabs(FR[3])) +
+                                    Smax * (fabs(hL.e.en) + fabs(hR.e.en))) &&
+                finite_in(f[4], fmax(fabs(FL[4]), fabs(FR[4])) + Smax * (fabs(hL.C) + fabs(hR.C)));
+            if (!ok) {
+                hll = true;
+            } else {
+                HlldAdj &aX = left ? aL : aR;
+                const double l0 = lamF[0], l1 = lamF[1], l2 = lamF[2], l3 = lamF[3], l4 = lamF[4];
+                double lS = 0.0, lSM = 0.0, lrs = 0.0, lvs = 0.0, lBs = 0.0, lens = 0.0;
+                double lSs = 0.0, lvss = 0.0, lBss = 0.0, lenss = 0.0;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) aX.lF[c] += lamF[c];
+                lS += l0 * (sX.rs - hX.r);
+                lrs += l0 * S;
+                aX.l[0] -= l0 * S;
+                lS += l1 * (sX.rs * SM - hX.r * hX.u);
+                lrs += l1 * S * SM;
+                lSM += l1 * S * sX.rs;
+                aX.l[0] -= l1 * S * hX.u;
+                aX.l[1] -= l1 * S * hX.r;
+                lS += l2 * (sX.rs * sX.vs - hX.r * hX.w);
+                lrs += l2 * S * sX.vs;
+                lvs += l2 * S * sX.rs;
+                aX.l[0] -= l2 * S * hX.w;
+                aX.l[2] -= l2 * S * hX.r;
+                lS += l3 * (sX.ens - hX.e.en);
+                lens += l3 * S;
+                aX.len -= l3 * S;
+                lS += l4 * (sX.Bs - hX.C);
+                lBs += l4 * S;
+                aX.l[5] -= l4 * S;
+                if (star2) {
+                    lSs += l2 * sX.rs * (vss - sX.vs);
+                    lrs += l2 * Ss * (vss - sX.vs);
+                    lvss += l2 * (Ss * sX.rs);
+                    lvs -= l2 * (Ss * sX.rs);
+                    lSs += l3 * (enXss - sX.ens);
+                    lenss += l3 * Ss;
+                    lens -= l3 * Ss;
+                    lSs += l4 * (Bss - sX.Bs);
+                    lBss += l4 * Ss;
+                    lBs -= l4 * Ss;
+                }
+                if (left) lSL += lS;
+                else lSR += lS;
+                // per-side star adjoints
+                double lrsL = left ? lrs : 0.0, lrsR = left ? 0.0 : lrs;
+                double lvsL = left ? lvs : 0.0, lvsR = left ? 0.0 : lvs;
+                double lBsL = left ? lBs : 0.0, lBsR = left ? 0.0 : lBs;
+                double lensL = left ? lens : 0.0, lensR = left ? 0.0 : lens;
+                double lvdotBsL = 0.0, lvdotBsR = 0.0, lsqL = 0.0, lsqR = 0.0;
+                double lSLs = left ? lSs : 0.0, lSRs = left ? 0.0 : lSs;
+                double lBavg = 0.0;
+                if (big) {
+                    double lvdotBss = 0.0;
+                    if (left) {   // enLss = enLs - sqL sgn (vdotBLs - vdotBss)
+                        lensL += lenss;
+                        lsqL -= lenss * sgn * (sL.vdotBs - vdotBss);
+                        lvdotBsL -= lenss * (sL.sq * sgn);
+                        lvdotBss += lenss * (sL.sq * sgn);
+                    } else {      // enRss = enRs + sqR sgn (vdotBRs - vdotBss)
+                        lensR += lenss;
+                        lsqR += lenss * sgn * (sR.vdotBs - vdotBss);
+                        lvdotBsR += lenss * (sR.sq * sgn);
+                        lvdotBss -= lenss * (sR.sq * sgn);
+                    }
+                    lSM += lvdotBss * Bavg;
+                    lBavg += lvdotBss * SM;
+                    lvss += lvdotBss * Bss;
+                    lBss += lvdotBss * vss;
+                    double ldss = 0.0;
+                    const double lnB = lBss / dss;
+                    ldss -= lBss * Bss / dss;
+                    const double dv = sR.vs - sL.vs;
+                    lsqL += lnB * sR.Bs;
+                    lBsR += lnB * sL.sq;
+                    lsqR += lnB * sL.Bs;
+                    lBsL += lnB * sR.sq;
+                    lsqL += lnB * sgn * sR.sq * dv;
+                    lsqR += lnB * sgn * sL.sq * dv;
+                    lvsR += lnB * (sgn * sL.sq * sR.sq);
+                    lvsL -= lnB * (sgn * sL.sq * sR.sq);
+                    const double lnv = lvss / dss;
+                    ldss -= lvss * vss / dss;
+                    lsqL += lnv * sL.vs;
+                    lvsL += lnv * sL.sq;
+                    lsqR += lnv * sR.vs;
+                    lvsR += lnv * sR.sq;
+                    lBsR += lnv * sgn;
+                    lBsL -= lnv * sgn;
+                    lsqL += ldss;
+                    lsqR += ldss;
+                    // SLs = SM - |Bavg| / sqL, SRs = SM + |Bavg| / sqR
+                    double labs = 0.0;
+                    lSM += lSLs;
+                    labs -= lSLs / sL.sq;
+                    lsqL += lSLs * (absB / sL.sq) / sL.sq;
+                    lSM += lSRs;
+                    labs += lSRs / sR.sq;
+                    lsqR -= lSRs * (absB / sR.sq) / sR.sq;
+                    lBavg += labs * sgn;
+                } else {
+                    lvsL += 0.5 * lvss;
+                    lvsR += 0.5 * lvss;
+                    lBsL += 0.5 * lBss;
+                    lBsR += 0.5 * lBss;
+                    if (left) lensL += lenss;
+                    else lensR += lenss;
+                    lSM += lSLs;
+                    lSM += lSRs;
+                }
+                aL.l[4] += 0.5 * lBavg;
+                aR.l[4] += 0.5 * lBavg;
+                double lpTs = 0.0, laL = 0.0, laR = 0.0;
+                hlld_star_rev(hL, sL, SL, SM, pTs, lrsL, lvsL, lBsL, lvdotBsL, lensL, lsqL, aL,
+                              lSL, lSM, lpTs, laL);
+                hlld_star_rev(hR, sR, SR, SM, pTs, lrsR, lvsR, lBsR, lvdotBsR, lensR, lsqR, aR,
+                              lSR, lSM, lpTs, laR);
+                // pTs = max(pTL + aL (SM - uL), P_floor)
+                const double lpTs0 = pts_floor ? 0.0 : lpTs;
+                aL.l[3] += lpTs0;
+                laL += lpTs0 * (SM - hL.u);
+                lSM += lpTs0 * aLv;
+                aL.l[1] -= lpTs0 * aLv;
+                // SM = num / denMD
+                const double lnum = lSM / denMD;
+                const double ldenMD = md_floor ? 0.0 : -lSM * SM / denMD;
+                aR.l[0] += lnum * hR.u * (SR - hR.u);
+                aR.l[1] += lnum * hR.r * (SR - hR.u);
+                lSR += lnum * (hR.r * hR.u);
+                aR.l[1] -= lnum * (hR.r * hR.u);
+                aL.l[0] -= lnum * hL.u * (SL - hL.u);
+                aL.l[1] -= lnum * hL.r * (SL - hL.u);
+                lSL -= lnum * (hL.r * hL.u);
+                aL.l[1] += lnum * (hL.r * hL.u);
+                aR.l[3] -= lnum;
+                aL.l[3] += lnum;
+                laR += ldenMD;
+                laL -= ldenMD;
+                // a = r (S - u)
+                aL.l[0] += laL * (SL - hL.u);
+                lSL += laL * hL.r;
+                aL.l[1] -= laL * hL.r;
+                aR.l[0] += laR * (SR - hR.u);
+                lSR += laR * hR.r;
+                aR.l[1] -= laR * hR.r;
+            }
+        }
+        if (hll) hll_avg_rev(hL, hR, FL, FR, SL, SR, lamF, aL, aR, lSL, lSR);
+    }
+    if (speeds) {
+        // SL = min(uL - cfL, uR - cfR), SR = max(uL + cfL, uR + cfR); no cap on this path
+        double lcfL = 0.0, lcfR = 0.0;
+        if (sl_right) {
+            aR.l[1] += lSL;
+            lcfR -= lSL;
+        } else {
+            aL.l[1] += lSL;
+            lcfL -= lSL;
+        }
+        if (sr_right) {
+            aR.l[1] += lSR;
+            lcfR += lSR;
+        } else {
+            aL.l[1] += lSR;
+            lcfL += lSR;
+        }
+        if (!capL) hlld_cf_rev(hL, g, lcfL, aL);
+        if (!capR) hlld_cf_rev(hR, g, lcfR, aR);
+    }
+    hlld_side_finish(L, hL, g, aL, lamL);
+    hlld_side_finish(R, hR, g, aR, lamR);
+}
+
+// Face VJP of the solver SOLVER, reverse mode (by hand) for all three solvers; the
+// forward-mode dual-number VJP (riemann_vjp, one pass per input) is the cross-check.
 template <int SOLVER>
 __device__ __forceinline__ void riemann_rev(const FaceState &L, const FaceState &R, double g,
                                             const double lamF[5], double lamL[6], double lamR[6]) {
     if (SOLVER == RUSANOV) rusanov_vjp(L, R, g, lamF, lamL, lamR);
     else if (SOLVER == HLLE) hlle_vjp(L, R, g, lamF, lamL, lamR);
-    else riemann_vjp<HLLD, CHIMERA_HLLD_DUAL_N>(L, R, g, lamF, lamL, lamR);
+    else hlld_vjp(L, R, g, lamF, lamL, lamR);
 }
 
 }  // namespace chimera

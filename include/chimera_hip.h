@@ -277,6 +277,15 @@ void *chimera_adj_stream(chimera_adj_t *adj);
  * counterpart in the reference (it has no derivatives). */
 int chimera_adj_set_fused(chimera_adj_t *adj, int fused);
 
+/* Riemann-solver VJPs of n x-direction faces (host arrays): in[12][n] holds the face
+ * states in the order of chimera_compute_fluxes (rho_L, rho_R, vx_L, vx_R, vy_L, vy_R,
+ * P_L, P_R, Bx_L, Bx_R, By_L, By_R), lam_flux[5][n] the adjoint of (mass, Momx, Momy,
+ * energy, By) flux, out[12][n] receives the adjoint of each input.  method 0: the
+ * hand-written reverse the adjoint uses, 1: forward-mode dual numbers (one pass per
+ * input), the cross-check of the first.  No counterpart in the reference. */
+int chimera_face_vjp(int solver, int method, int n, double gamma, const double *in,
+                     const double *lam_flux, double *out);
+
 #ifdef __cplusplus
 }
 #endif
