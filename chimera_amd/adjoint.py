@@ -45,6 +45,11 @@ class AdjointStepper:
         both give the same bits."""
         check(self._L.chimera_adj_set_fused(self._h, int(bool(fused))), "chimera_adj_set_fused")
 
+    def set_tiles(self, shape_a: int = -1, shape_b: int = -1) -> None:
+        """Tile shapes of the fused reverse kernels: 0 = 16x16, 1 = 16x8, 2 = 8x8, 3 = 32x8."""
+        check(self._L.chimera_adj_set_tiles(self._h, int(shape_a), int(shape_b)),
+              "chimera_adj_set_tiles")
+
     def forward_step(self, x: torch.Tensor, dt: float, out: torch.Tensor | None = None):
         out = torch.empty_like(x) if out is None else out
         torch.cuda.synchronize()

@@ -53,7 +53,13 @@ __device__ __forceinline__ int wrap_or_clamp(int g, int n, int periodic) {
 // ---- forward recompute: the fused stage kernel, keeping the intermediates -----------
 
 constexpr int ATX = 32, ATY = 8;
-constexpr int RTX = 16, RTY = 16;   // tiles of the fused reverse kernels
+// tile shapes of the fused reverse kernels (chimera_adj_set_tiles); smaller tiles give
+// more workgroups on small grids at the price of more halo work
+struct RevTile {
+    int tx, ty;
+};
+constexpr int REV_NUM_SHAPES = 4;
+constexpr RevTile REV_SHAPES[REV_NUM_SHAPES] = {{16, 16}, {16, 8}, {8, 8}, {32, 8}};
 
 template <int SOLVER, int LIMITER, bool STORE>
 __global__ __launch_bounds__(STAGE_THREADS) void adj_stage_kernel(GridView g, StageStore st,
@@ -443,6 +449,31 @@ __global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, Stage
     const double inv_dx = 1.0 / dx, inv_2dx = 1.0 / (2.0 * dx), hdx = 0.5 * dx;
     const bool use_lim = G.use_lim != 0, mood = G.mood != 0;
 
+    // loads of phase 3 (the stage input, u^n and lam_out of cells [0, TX] x [0, TY]),
+    // issued first so that they arrive while phases 1 and 2 compute
+    constexpr int CPT = (A::C_SZ + NT - 1) / NT;
+    const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
+    const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
+    size_t ck[CPT];
+    bool cvalid[CPT];
+    double cin[CPT][4], cu0[CPT][4], clo[CPT][6];
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const int c = tid + u * NT;
+        const int gx = x0 + c % A::CW, gy = y0 + c / A::CW;
+        cvalid[u] = c < A::C_SZ && !((!px && gx >= nx) || (!py && gy >= ny));
+        ck[u] = cvalid[u] ? (size_t)map_index(gx, nx, px) + (size_t)nx * map_index(gy, ny, py) : 0;
+        if (cvalid[u]) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                cin[u][q] = g.uin[q][ck[u]];
+                cu0[u][q] = stage > 1 ? g.u0[q][ck[u]] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) clo[u][q] = lout[q][ck[u]];
+        }
+    }
+
     // ---- 1: primitives of cells [-2, TX+2] x [-2, TY+2] ------------------------------
     for (int idx = tid; idx < A::P_SZ; idx += NT) {
         const int lx = idx % A::PW, ly = idx / A::PW;
@@ -509,18 +540,19 @@ __global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, Stage
 
     // ---- 3: forward tail and its reverse for cells [0, TX] x [0, TY] ------------------
     const double dtdx = dt * dx;
-    const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
-    const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
-    for (int c = tid; c < A::C_SZ; c += NT) {
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const int c = tid + u * NT;
+        if (c >= A::C_SZ) break;
         const int ci = c % A::CW, cj = c / A::CW;
         const int gx = x0 + ci, gy = y0 + cj;
-        if ((!px && gx >= nx) || (!py && gy >= ny)) {
+        if (!cvalid[u]) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) sLU[q * A::C_SZ + c] = 0.0;
             continue;
         }
         const int gi = map_index(gx, nx, px), gj = map_index(gy, ny, py);
-        const size_t k = (size_t)gi + (size_t)nx * gj;
+        const size_t k = ck[u];
         const int xr = cj * A::FXW + (ci + 1);         // x face right of the cell
         const int yr = (cj + 1) * A::FYW + ci;         // y face above the cell
         const bool xlo_open = (gi == 0) && !px;
@@ -528,7 +560,7 @@ __global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, Stage
         double Up[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            double v = g.uin[q][k];
+            double v = cin[u][q];
             const double fr = sFX[q * A::FX_SZ + xr];
             if (xlo_open) v = v - dtdx * fr;
             else v = v - dtdx * fr + dtdx * sFX[q * A::FX_SZ + xr - 1];
@@ -569,25 +601,32 @@ __global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, Stage
             double V[4], lo[4], lV[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                V[q] = c0 * g.u0[q][k] + c1 * U3[q];
-                lo[q] = lout[q][k];
+                V[q] = c0 * cu0[u][q] + c1 * U3[q];
+                lo[q] = clo[u][q];
             }
             floors_vjp(V, Bx, By, vol, gam, lo, lV, lBx, lBy);
+            // stage 3 starts lam_u0 (written), stage 2 adds to it
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                if (owned) lU0[q][k] += c0 * lV[q];
+                if (owned) {
+                    if (stage == 3) lU0[q][k] = c0 * lV[q];
+                    else lU0[q][k] += c0 * lV[q];
+                }
                 l3[q] = c1 * lV[q];
             }
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                if (owned) lU0[4 + q][k] += c0 * lout[4 + q][k];
-                lb[q] = c1 * lout[4 + q][k];
+                if (owned) {
+                    if (stage == 3) lU0[4 + q][k] = c0 * clo[u][4 + q];
+                    else lU0[4 + q][k] += c0 * clo[u][4 + q];
+                }
+                lb[q] = c1 * clo[u][4 + q];
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) l3[q] = lout[q][k];
+            for (int q = 0; q < 4; ++q) l3[q] = clo[u][q];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) lb[q] = lout[4 + q][k];
+            for (int q = 0; q < 2; ++q) lb[q] = clo[u][4 + q];
         }
         double lPd[6] = {0, 0, 0, 0, 0, 0};
         if (xlo_open) outflow_vjp(0, P6, gam, dtdx, l3, lPd);
@@ -648,56 +687,61 @@ __global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, Stage
     }
     __syncthreads();
 
-    // ---- 5: owned faces (as ar_faces) -------------------------------------------------
+    // ---- 5: owned faces (as ar_faces): per owned cell its x face, then its y face ------
     {
         const int ci = tid % TX, cj = tid / TX;
         const int gi = x0 + ci, gj = y0 + cj;
-        if (gi < nx && gj < ny) {
-            const size_t k = (size_t)gi + (size_t)nx * gj;
+        const bool active = gi < nx && gj < ny;
+        const size_t k = active ? (size_t)gi + (size_t)nx * gj : 0;
+        // lam of the two face states of the face dir of this cell, and its bits
+        auto face_rev = [&](int dir, double lL[6], double lR[6]) -> unsigned long long {
             auto lEz = [&](int a, int bb) { return -sLE[(bb + 1) * A::EW + (a + 1)]; };
             const double *b0 = sP + (cj + 2) * A::PW + (ci + 2);
+            const bool lastf = dir ? (!py && gj == ny - 1) : (!px && gi == nx - 1);
+            double L[6], Rr[6];
+            const unsigned long long bits =
+                face_pair<LIMITER>(b0, A::P_SZ, dir ? A::PW : 1, dir ? 1 : A::PW, lastf, mood,
+                                   use_lim, inv_dx, inv_2dx, hdx, dir, L, Rr);
+            const bool has_next = dir ? (gj < ny - 1 || py) : (gi < nx - 1 || px);
+            const int cn = dir ? (cj + 1) * A::CW + ci : cj * A::CW + ci + 1;
+            const int cc = cj * A::CW + ci;
+            double lF[5];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double v = -dtdx * sLU[q * A::C_SZ + cc];
+                if (has_next) v += dtdx * sLU[q * A::C_SZ + cn];
+                lF[q] = v;
+            }
+            if (!dir) {
+                double v = -0.25 * lEz(ci, cj);
+                if (gj >= 1 || py) v += -0.25 * lEz(ci, cj - 1);
+                if (!py && gj == ny - 1) v += -0.25 * lEz(ci, cj);
+                lF[4] = v;
+            } else {
+                double v = 0.25 * lEz(ci, cj);
+                if (gi >= 1 || px) v += 0.25 * lEz(ci - 1, cj);
+                if (!px && gi == nx - 1) v += 0.25 * lEz(ci, cj);
+                lF[4] = v;
+            }
+            const int fu = dir ? 2 : 1, fw = dir ? 1 : 2, fbn = dir ? 5 : 4, fbt = dir ? 4 : 5;
+            FaceState fa{L[0], L[fu], L[fw], L[3], L[fbn], L[fbt]};
+            FaceState fb{Rr[0], Rr[fu], Rr[fw], Rr[3], Rr[fbn], Rr[fbt]};
+            double lFr[5] = {lF[0], dir ? lF[2] : lF[1], dir ? lF[1] : lF[2], lF[3], lF[4]};
+            double lLr[6], lRr[6];
+            riemann_rev<SOLVER>(fa, fb, gam, lFr, lLr, lRr);
+            lL[0] = lLr[0]; lL[fu] = lLr[1]; lL[fw] = lLr[2]; lL[3] = lLr[3]; lL[fbn] = lLr[4]; lL[fbt] = lLr[5];
+            lR[0] = lRr[0]; lR[fu] = lRr[1]; lR[fw] = lRr[2]; lR[3] = lRr[3]; lR[fbn] = lRr[4]; lR[fbt] = lRr[5];
+            return bits;
+        };
+        if (active) {
             double lreset[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int dir = 0; dir < 2; ++dir) {
-                // the face states again, with this direction's bits (the x and y bits
-                // occupy disjoint positions of the stored words)
-                double L[6], Rr[6];
-                const bool lastf = dir ? (!py && gj == ny - 1) : (!px && gi == nx - 1);
-                const unsigned long long bits = face_pair<LIMITER>(
-                    b0, A::P_SZ, dir ? A::PW : 1, dir ? 1 : A::PW, lastf, mood, use_lim, inv_dx,
-                    inv_2dx, hdx, dir, L, Rr);
-                const bool has_next = dir ? (gj < ny - 1 || py) : (gi < nx - 1 || px);
-                const int cn = dir ? (cj + 1) * A::CW + ci : cj * A::CW + ci + 1;
-                const int cc = cj * A::CW + ci;
-                double lF[5];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    double v = -dtdx * sLU[q * A::C_SZ + cc];
-                    if (has_next) v += dtdx * sLU[q * A::C_SZ + cn];
-                    lF[q] = v;
-                }
-                if (!dir) {
-                    double v = -0.25 * lEz(ci, cj);
-                    if (gj >= 1 || py) v += -0.25 * lEz(ci, cj - 1);
-                    if (!py && gj == ny - 1) v += -0.25 * lEz(ci, cj);
-                    lF[4] = v;
-                } else {
-                    double v = 0.25 * lEz(ci, cj);
-                    if (gi >= 1 || px) v += 0.25 * lEz(ci - 1, cj);
-                    if (!px && gi == nx - 1) v += 0.25 * lEz(ci, cj);
-                    lF[4] = v;
-                }
-                const int fu = dir ? 2 : 1, fw = dir ? 1 : 2, fbn = dir ? 5 : 4, fbt = dir ? 4 : 5;
-                FaceState fa{L[0], L[fu], L[fw], L[3], L[fbn], L[fbt]};
-                FaceState fb{Rr[0], Rr[fu], Rr[fw], Rr[3], Rr[fbn], Rr[fbt]};
-                double lFr[5] = {lF[0], dir ? lF[2] : lF[1], dir ? lF[1] : lF[2], lF[3], lF[4]};
-                double lLr[6], lRr[6];
-                riemann_rev<SOLVER>(fa, fb, gam, lFr, lLr, lRr);
                 double lL[6], lR[6];
-                lL[0] = lLr[0]; lL[fu] = lLr[1]; lL[fw] = lLr[2]; lL[3] = lLr[3]; lL[fbn] = lLr[4]; lL[fbt] = lLr[5];
-                lR[0] = lRr[0]; lR[fu] = lRr[1]; lR[fw] = lRr[2]; lR[3] = lRr[3]; lR[fbn] = lRr[4]; lR[fbt] = lRr[5];
-                face_state_reverse(lL, dir ? 2 : 0, false, bits, lreset, dir ? R.lYL : R.lXL, k);
-                face_state_reverse(lR, dir ? 3 : 1, lastf, bits, lreset, dir ? R.lYR : R.lXR, k);
+                const unsigned long long bits = face_rev(dir, lL, lR);
+                const bool lastf = dir ? (!py && gj == ny - 1) : (!px && gi == nx - 1);
+                face_state_reverse(lL, 2 * dir, false, bits, lreset, dir ? R.lYL : R.lXL, k);
+                face_state_reverse(lR, 2 * dir + 1, lastf, bits, lreset, dir ? R.lYR : R.lXR, k);
             }
 #pragma unroll
             for (int f = 0; f < 6; ++f) R.lPr[f][k] = lreset[f];
@@ -714,9 +758,11 @@ struct RevShapeB {
     static constexpr int CPT = (C_SZ + TX * TY - 1) / (TX * TY);     // cells [0, T] per thread
 };
 
+// lU0_add: the step's lam_u0, added to lam_in at the end (stage 1), else nullptr
 template <int LIMITER, int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void ar_tile_b(AdjGrid G, const double *const *U, StageStore S,
-                                                   RevBufs R, double *const *lin, int tiles_x,
+                                                   RevBufs R, double *const *lin,
+                                                   const double *const *lU0_add, int tiles_x,
                                                    int ntiles) {
     using B = RevShapeB<TX, TY>;
     constexpr int NT = TX * TY;
@@ -741,45 +787,93 @@ __global__ __launch_bounds__(TX *TY) void ar_tile_b(AdjGrid G, const double *con
     };
     double lp6[B::CPT][6];
 
+    // per-thread positions of the primitive items and of the slope items, fixed over the
+    // six fields; each field's loads are issued one round ahead of its use
+    constexpr int PPT = (B::P_SZ + NT - 1) / NT, GPT = (B::G_SZ + NT - 1) / NT;
+    size_t kp[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const int idx = tid + u * NT;
+        const int gi = map_index(x0 - 2 + idx % B::PW, nx, px);
+        const int gj = map_index(y0 - 2 + idx / B::PW, ny, py);
+        kp[u] = idx < B::P_SZ ? (size_t)gi + (size_t)nx * gj : 0;
+    }
+    struct GItem {
+        size_t k, kl, kb;
+        bool valid, hl, hb, cxm, cxp, cym, cyp;
+    };
+    GItem gt[GPT];
+#pragma unroll
+    for (int u = 0; u < GPT; ++u) {
+        const int idx = tid + u * NT;
+        GItem &t = gt[u];
+        int i = 0, j = 0;
+        t.valid = idx < B::G_SZ && gcell(idx % B::GW - 1, idx / B::GW - 1, i, j);
+        t.k = (size_t)i + (size_t)nx * j;
+        t.hl = i > 0 || px;
+        t.hb = j > 0 || py;
+        t.kl = i > 0 ? t.k - 1 : t.k + nx - 1;
+        t.kb = j > 0 ? t.k - nx : t.k + (size_t)nx * (ny - 1);
+        t.cxm = !px && i == 0;   // clamped neighbours
+        t.cxp = !px && i == nx - 1;
+        t.cym = !py && j == 0;
+        t.cyp = !py && j == ny - 1;
+    }
+    double pv[2][PPT], rv[2][GPT][6];
+    auto load = [&](int f, int buf) {
+#pragma unroll
+        for (int u = 0; u < PPT; ++u)
+            if (tid + u * NT < B::P_SZ) pv[buf][u] = S.P[f][kp[u]];
+#pragma unroll
+        for (int u = 0; u < GPT; ++u) {
+            const GItem &t = gt[u];
+            if (!t.valid) continue;
+            rv[buf][u][0] = R.lXL[f][t.k];
+            rv[buf][u][1] = t.hl ? R.lXR[f][t.kl] : 0.0;   // right state of face i-1/2
+            rv[buf][u][2] = R.lYL[f][t.k];
+            rv[buf][u][3] = t.hb ? R.lYR[f][t.kb] : 0.0;
+            rv[buf][u][4] = R.lPr[f][t.k];
+            rv[buf][u][5] = R.lPd[f][t.k];
+        }
+    };
+    load(0, 0);
+
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
+        const int cur = f & 1;
         // the field's stage primitive over cells [-2, T+2] (clamped like the forward)
-        for (int idx = tid; idx < B::P_SZ; idx += NT) {
-            const int gi = map_index(x0 - 2 + idx % B::PW, nx, px);
-            const int gj = map_index(y0 - 2 + idx / B::PW, ny, py);
-            sA[idx] = S.P[f][(size_t)gi + (size_t)nx * gj];
-        }
+#pragma unroll
+        for (int u = 0; u < PPT; ++u)
+            if (tid + u * NT < B::P_SZ) sA[tid + u * NT] = pv[cur][u];
         __syncthreads();
+        if (f < 5) load(f + 1, cur ^ 1);
         // slope parts of cells [-1, T+1] (as ar_slopes)
-        for (int idx = tid; idx < B::G_SZ; idx += NT) {
-            const int lx = idx % B::GW - 1, ly = idx / B::GW - 1;
-            int i, j;
-            if (!gcell(lx, ly, i, j)) {
+#pragma unroll
+        for (int u = 0; u < GPT; ++u) {
+            const int idx = tid + u * NT;
+            if (idx >= B::G_SZ) break;
+            const GItem &t = gt[u];
+            if (!t.valid) {
 #pragma unroll
                 for (int r = 0; r < 5; ++r) sG[r * B::G_SZ + idx] = 0.0;
                 continue;
             }
-            const size_t k = (size_t)i + (size_t)nx * j;
-            const bool hl = i > 0 || px, hb = j > 0 || py;
-            const size_t kl = i > 0 ? k - 1 : k + nx - 1;
-            const size_t kb = j > 0 ? k - nx : k + (size_t)nx * (ny - 1);
-            const bool cxm = !px && i == 0, cxp = !px && i == nx - 1;   // clamped neighbours
-            const bool cym = !py && j == 0, cyp = !py && j == ny - 1;
+            const int lx = idx % B::GW - 1, ly = idx / B::GW - 1;
             const double *a = sA + (ly + 2) * B::PW + (lx + 2);
-            const double am = cxm ? a[0] : a[-1], ap = cxp ? a[0] : a[1];
-            const double bm = cym ? a[0] : a[-B::PW], bp = cyp ? a[0] : a[B::PW];
-            const double xr = hl ? R.lXR[f][kl] : 0.0;
-            const double yr = hb ? R.lYR[f][kb] : 0.0;
-            const double lSx = hdx * (R.lXL[f][k] - xr);
-            const double lSy = hdx * (R.lYL[f][k] - yr);
-            double lp = R.lXL[f][k] + xr + R.lYL[f][k] + yr + R.lPr[f][k] + R.lPd[f][k];
+            const double am = t.cxm ? a[0] : a[-1], ap = t.cxp ? a[0] : a[1];
+            const double bm = t.cym ? a[0] : a[-B::PW], bp = t.cyp ? a[0] : a[B::PW];
+            const double *r6 = rv[cur][u];
+            const double xr = r6[1], yr = r6[3];
+            const double lSx = hdx * (r6[0] - xr);
+            const double lSy = hdx * (r6[2] - yr);
+            double lp = r6[0] + xr + r6[2] + yr + r6[4] + r6[5];
             double dm, d0, dp;
             if (G.use_lim) limited_slope_grad<LIMITER>(am, a[0], ap, inv_dx, dm, d0, dp);
             else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
             double gm = lSx * dm, gp = lSx * dp;
             lp += lSx * d0;
-            if (cxm) { lp += gm; gm = 0.0; }
-            if (cxp) { lp += gp; gp = 0.0; }
+            if (t.cxm) { lp += gm; gm = 0.0; }
+            if (t.cxp) { lp += gp; gp = 0.0; }
             sG[0 * B::G_SZ + idx] = gm;
             sG[1 * B::G_SZ + idx] = gp;
             if (G.use_lim) limited_slope_grad<LIMITER>(bm, a[0], bp, inv_dx, dm, d0, dp);
@@ -787,8 +881,8 @@ __global__ __launch_bounds__(TX *TY) void ar_tile_b(AdjGrid G, const double *con
             gm = lSy * dm;
             gp = lSy * dp;
             lp += lSy * d0;
-            if (cym) { lp += gm; gm = 0.0; }
-            if (cyp) { lp += gp; gp = 0.0; }
+            if (t.cym) { lp += gm; gm = 0.0; }
+            if (t.cyp) { lp += gp; gp = 0.0; }
             sG[2 * B::G_SZ + idx] = gm;
             sG[3 * B::G_SZ + idx] = gp;
             sG[4 * B::G_SZ + idx] = lp;
@@ -857,6 +951,10 @@ __global__ __launch_bounds__(TX *TY) void ar_tile_b(AdjGrid G, const double *con
             if (j + 1 < ny || py) ay += 0.5 * lby[c + B::CW];
             lin[4][k] += ax;
             lin[5][k] += ay;
+            if (lU0_add) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) lin[q][k] += lU0_add[q][k];
+            }
         }
     }
 }
@@ -912,7 +1010,7 @@ struct Ptr6 {
 struct chimera_adj {
     AdjGrid G{};
     bool fused = true;                        // fused reverse (ar_tile_a / ar_tile_b)
-    int rtiles_x = 0, rntiles = 0, rgrid = 0; // their launch shape (RTX x RTY tiles)
+    int rshape_a = 3, rshape_b = 3;           // their tile shapes (REV_SHAPES)
     chimera_scheme_t sch{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -1020,32 +1118,67 @@ int run_forward_stage(chimera_adj *a, int stage, const Ptr6 &in, const Ptr6 &u0,
 
 using TileAFn = void (*)(GridView, AdjGrid, StageBufs, RevBufs, const double *const *,
                          double *const *, double *const *, int, double, int, int);
-using TileBFn = void (*)(AdjGrid, const double *const *, StageStore, RevBufs, double *const *, int,
-                         int);
+using TileBFn = void (*)(AdjGrid, const double *const *, StageStore, RevBufs, double *const *,
+                         const double *const *, int, int);
 
-TileAFn tile_a_fn(int solver, int limiter) {
-    static const TileAFn t[3][2] = {
-        {ar_tile_a<RUSANOV, LIM_VAN_LEER, RTX, RTY>, ar_tile_a<RUSANOV, LIM_MC, RTX, RTY>},
-        {ar_tile_a<HLLE, LIM_VAN_LEER, RTX, RTY>, ar_tile_a<HLLE, LIM_MC, RTX, RTY>},
-        {ar_tile_a<HLLD, LIM_VAN_LEER, RTX, RTY>, ar_tile_a<HLLD, LIM_MC, RTX, RTY>}};
-    return t[solver - 1][limiter - 1];
+template <int TX, int TY>
+constexpr TileAFn tile_a_of(int solver, int limiter) {
+    return solver == RUSANOV ? (limiter == LIM_MC ? ar_tile_a<RUSANOV, LIM_MC, TX, TY>
+                                                  : ar_tile_a<RUSANOV, LIM_VAN_LEER, TX, TY>)
+         : solver == HLLE    ? (limiter == LIM_MC ? ar_tile_a<HLLE, LIM_MC, TX, TY>
+                                                  : ar_tile_a<HLLE, LIM_VAN_LEER, TX, TY>)
+                             : (limiter == LIM_MC ? ar_tile_a<HLLD, LIM_MC, TX, TY>
+                                                  : ar_tile_a<HLLD, LIM_VAN_LEER, TX, TY>);
+}
+
+TileAFn tile_a_fn(int solver, int limiter, int shape) {
+    switch (shape) {
+    case 0: return tile_a_of<16, 16>(solver, limiter);
+    case 1: return tile_a_of<16, 8>(solver, limiter);
+    case 2: return tile_a_of<8, 8>(solver, limiter);
+    default: return tile_a_of<32, 8>(solver, limiter);
+    }
+}
+
+TileBFn tile_b_fn(int limiter, int shape) {
+    const bool mc = limiter == LIM_MC;
+    switch (shape) {
+    case 0: return mc ? ar_tile_b<LIM_MC, 16, 16> : ar_tile_b<LIM_VAN_LEER, 16, 16>;
+    case 1: return mc ? ar_tile_b<LIM_MC, 16, 8> : ar_tile_b<LIM_VAN_LEER, 16, 8>;
+    case 2: return mc ? ar_tile_b<LIM_MC, 8, 8> : ar_tile_b<LIM_VAN_LEER, 8, 8>;
+    default: return mc ? ar_tile_b<LIM_MC, 32, 8> : ar_tile_b<LIM_VAN_LEER, 32, 8>;
+    }
+}
+
+// tiles_x, ntiles and the (multiple of 8) grid of one reverse tile shape
+void rev_launch_shape(const chimera_adj *a, int shape, int &tiles_x, int &ntiles, int &grid) {
+    const RevTile t = REV_SHAPES[shape];
+    tiles_x = (a->G.nx + t.tx - 1) / t.tx;
+    ntiles = tiles_x * ((a->G.ny + t.ty - 1) / t.ty);
+    grid = (ntiles + 7) / 8 * 8;
 }
 
 // The reverse of one stage through the fused kernels: in is the stage input (host
-// pointer set and device table tin), u0 the step input.
+// pointer set and device table tin), u0 the step input.  Stage 3 starts lam_u0,
+// stage 2 adds to it and stage 1 adds it to lam_in.
 int run_reverse_stage_fused(chimera_adj *a, int stage, const Ptr6 &in, Tab tin, const Ptr6 &u0,
                             Tab tlout, Tab tlin, double dt) {
     GridView g = grid_view(a, in, u0, in, nullptr);
-    hipLaunchKernelGGL(tile_a_fn(a->sch.riemann_solver, a->sch.slope_limiter), dim3(a->rgrid),
-                       dim3(RTX * RTY), 0, a->stream, g, a->G, a->S[stage - 1], a->R, tlout, tlin,
-                       table(a, TB_LU0), stage, dt, a->rtiles_x, a->rntiles);
-    TileBFn fb = a->sch.slope_limiter == LIM_MC ? ar_tile_b<LIM_MC, RTX, RTY>
-                                                : ar_tile_b<LIM_VAN_LEER, RTX, RTY>;
-    hipLaunchKernelGGL(fb, dim3(a->rgrid), dim3(RTX * RTY), 0, a->stream, a->G, tin, a->S[stage - 1],
-                       a->R, tlin, a->rtiles_x, a->rntiles);
+    int tx, nt, grid;
+    rev_launch_shape(a, a->rshape_a, tx, nt, grid);
+    const RevTile ta = REV_SHAPES[a->rshape_a];
+    hipLaunchKernelGGL(tile_a_fn(a->sch.riemann_solver, a->sch.slope_limiter, a->rshape_a),
+                       dim3(grid), dim3(ta.tx * ta.ty), 0, a->stream, g, a->G, a->S[stage - 1],
+                       a->R, tlout, tlin, table(a, TB_LU0), stage, dt, tx, nt);
+    rev_launch_shape(a, a->rshape_b, tx, nt, grid);
+    const RevTile tb = REV_SHAPES[a->rshape_b];
+    hipLaunchKernelGGL(tile_b_fn(a->sch.slope_limiter, a->rshape_b), dim3(grid),
+                       dim3(tb.tx * tb.ty), 0, a->stream, a->G, tin, a->S[stage - 1], a->R, tlin,
+                       stage == 1 ? (const double *const *)table(a, TB_LU0) : nullptr, tx, nt);
     return launch_check("adjoint fused reverse stage");
 }
 
+// The reverse of one stage through the stored-intermediate kernels.
 int run_reverse_stage(chimera_adj *a, int stage, Tab tin, Tab tu0, Tab tlout, Tab tlin, double dt) {
     StageBufs &S = a->S[stage - 1];
     RevBufs &R = a->R;
@@ -1087,13 +1220,9 @@ int vjp_core(chimera_adj *a, const Ptr6 &hs, Tab ts, const double *dt_dev, doubl
             t1 = table(a, TB_S1);
             t2 = table(a, TB_S2);
         }
-        hipLaunchKernelGGL(zero6_kernel, dim3(nb), dim3(BS), 0, a->stream, table(a, TB_LU0), a->n);
         CH_TRY(run_reverse_stage_fused(a, 3, *s2, t2, hs, tlout, table(a, TB_LB), dt));
         CH_TRY(run_reverse_stage_fused(a, 2, *s1, t1, hs, table(a, TB_LB), table(a, TB_LA), dt));
-        CH_TRY(run_reverse_stage_fused(a, 1, hs, ts, hs, table(a, TB_LA), tlin, dt));
-        hipLaunchKernelGGL(add_kernel, dim3(nb), dim3(BS), 0, a->stream, tlin,
-                           (const double *const *)table(a, TB_LU0), a->n);
-        return launch_check("adjoint accumulate");
+        return run_reverse_stage_fused(a, 1, hs, ts, hs, table(a, TB_LA), tlin, dt);
     }
     // forward: keep every stage's intermediates
     CH_TRY(run_forward_stage(a, 1, hs, hs, a->stage_state[1], dt_dev));
@@ -1167,9 +1296,6 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
     a->tiles_x = (nx + ATX - 1) / ATX;
     a->ntiles = a->tiles_x * ((ny + ATY - 1) / ATY);
     a->grid = (a->ntiles + 7) / 8 * 8;
-    a->rtiles_x = (nx + RTX - 1) / RTX;
-    a->rntiles = a->rtiles_x * ((ny + RTY - 1) / RTY);
-    a->rgrid = (a->rntiles + 7) / 8 * 8;
     if (stream) {
         a->stream = (hipStream_t)stream;
     } else {
@@ -1329,6 +1455,18 @@ extern "C" int chimera_adj_segment_vjp(chimera_adj_t *a, const double *const *st
 }
 
 extern "C" void *chimera_adj_stream(chimera_adj_t *a) { return a ? (void *)a->stream : nullptr; }
+
+// Tile shapes of the two fused reverse kernels (indices into REV_SHAPES; -1 keeps).
+extern "C" int chimera_adj_set_tiles(chimera_adj_t *a, int shape_a, int shape_b) {
+    CH_REQUIRE(a, "NULL adjoint");
+    CH_REQUIRE(shape_a >= -1 && shape_a < REV_NUM_SHAPES && shape_b >= -1 &&
+                   shape_b < REV_NUM_SHAPES,
+               "tile shapes must be -1 or 0..%d, got %d, %d", REV_NUM_SHAPES - 1, shape_a, shape_b);
+    CH_HIP(hipStreamSynchronize(a->stream));
+    if (shape_a >= 0) a->rshape_a = shape_a;
+    if (shape_b >= 0) a->rshape_b = shape_b;
+    return CHIMERA_OK;
+}
 
 // 1 (default): the fused reverse kernels; 0: the stored-intermediate kernels.  Both
 // give the same bits; the second is kept as the cross-check of the first.

@@ -276,6 +276,9 @@ void *chimera_adj_stream(chimera_adj_t *adj);
  * stored-intermediate kernels.  Both give the same bits; 0 is the cross-check.  No
  * counterpart in the reference (it has no derivatives). */
 int chimera_adj_set_fused(chimera_adj_t *adj, int fused);
+/* Tile shapes of the two fused reverse kernels: 0 = 16x16, 1 = 16x8, 2 = 8x8, 3 = 32x8
+ * (-1 keeps the current one).  Every shape gives the same bits. */
+int chimera_adj_set_tiles(chimera_adj_t *adj, int shape_a, int shape_b);
 
 /* Riemann-solver VJPs of n x-direction faces (host arrays): in[12][n] holds the face
  * states in the order of chimera_compute_fluxes (rho_L, rho_R, vx_L, vx_R, vy_L, vy_R,

@@ -259,12 +259,15 @@ def test_fused_reverse_matches_stored_intermediates(gpu_lib, name, shape, scheme
     lam = torch.from_numpy(np.random.default_rng(5).standard_normal(p.state.shape)).cuda()
     x = torch.from_numpy(p.state.copy()).cuda()
     adj = AdjointStepper(p.nx, p.ny, p.dx, p.gamma, p.bc, p.scheme)
-    a = adj.vjp(x, dt, lam).cpu().numpy()
     adj.set_fused(False)
     b = adj.vjp(x, dt, lam).cpu().numpy()
+    adj.set_fused(True)
+    for tiles in [(0, 0), (1, 2), (2, 3), (3, 1)]:   # every reverse tile shape
+        adj.set_tiles(*tiles)
+        a = adj.vjp(x, dt, lam).cpu().numpy()
+        assert np.all(np.isfinite(a))
+        assert np.array_equal(a, b), (tiles, float(np.max(np.abs(a - b))))
     adj.close()
-    assert np.all(np.isfinite(a))
-    assert np.array_equal(a, b), float(np.max(np.abs(a - b)))
 
 
 def _face_vjp(solver, method, fin, lam, gamma=1.4):
