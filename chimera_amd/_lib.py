@@ -134,11 +134,18 @@ def _declare(L: C.CDLL) -> None:
         "chimera_adj_set_fused": [vp, i],
         "chimera_adj_set_tiles": [vp, i, i],
         "chimera_face_vjp": [i, i, i, C.c_double, c_dp, c_dp, c_dp],
+        "chimera_badj_create": [C.POINTER(vp), i, C.POINTER(C.c_int), C.POINTER(C.c_int), c_dp, c_dp,
+                                vp, vp, vp],
+        "chimera_badj_destroy": [vp],
+        "chimera_badj_forward": [vp, C.POINTER(vp), c_dp, i, C.POINTER(vp)],
+        "chimera_badj_segment_vjp": [vp, C.POINTER(vp), c_dp, i, C.POINTER(vp)],
+        "chimera_badj_set_tiles": [vp, i, i],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = C.c_int
-    for name in ("chimera_ctx_stream", "chimera_batch_stream", "chimera_adj_stream"):
+    for name in ("chimera_ctx_stream", "chimera_batch_stream", "chimera_adj_stream",
+                 "chimera_badj_stream"):
         getattr(L, name).argtypes = [vp]
         getattr(L, name).restype = vp

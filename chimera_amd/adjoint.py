@@ -147,6 +147,109 @@ def evolve(U0: torch.Tensor, dts, prob_or_geom, checkpoint_every: int = 50) -> t
     return _Evolve.apply(U0, dts, geom, checkpoint_every)
 
 
+# ---- batched ensembles ---------------------------------------------------------
+
+def _ptrs_many(ts) -> "C.Array":
+    for t in ts:
+        assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.shape[0] == 6
+    return (C.c_void_p * (6 * len(ts)))(*[t[q].data_ptr() for t in ts for q in range(6)])
+
+
+def _dt_table(dts, members: int) -> np.ndarray:
+    d = np.ascontiguousarray(dts, dtype=np.float64)
+    assert d.ndim == 2 and d.shape[0] == members, d.shape
+    return d
+
+
+class BatchAdjoint:
+    """chimera_badj_t: forward steps and checkpoint-segment VJPs of many grids of uneven
+    sizes at once (one persistent launch per stage over all members).  geoms: one
+    (nx, ny, dx, gamma, bc) per member; the scheme is shared."""
+
+    def __init__(self, geoms, scheme):
+        L = _lib.lib()
+        self._L = L
+        self.members = len(geoms)
+        self.shapes = [(int(g[1]), int(g[0])) for g in geoms]   # (ny, nx)
+        nx = (C.c_int * self.members)(*[int(g[0]) for g in geoms])
+        ny = (C.c_int * self.members)(*[int(g[1]) for g in geoms])
+        dx = (C.c_double * self.members)(*[float(g[2]) for g in geoms])
+        gm = (C.c_double * self.members)(*[float(g[3]) for g in geoms])
+        bcs = (BC * self.members)(*[BC.of(g[4]) for g in geoms])
+        sch = Scheme.of(scheme)
+        h = C.c_void_p()
+        check(L.chimera_badj_create(C.byref(h), self.members, nx, ny, dx, gm, C.cast(bcs, C.c_void_p),
+                                    C.cast(C.pointer(sch), C.c_void_p), None), "chimera_badj_create")
+        self._h = h
+
+    def forward(self, states, dts, out=None):
+        """Every member len(dts[m]) steps with its dts, from states to out (new tensors
+        unless given; may be the inputs)."""
+        d = _dt_table(dts, self.members)
+        out = [torch.empty_like(x) for x in states] if out is None else out
+        torch.cuda.synchronize()
+        check(self._L.chimera_badj_forward(self._h, _ptrs_many(states),
+                                           d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
+                                           _ptrs_many(out)), "chimera_badj_forward")
+        return out
+
+    def segment_vjp(self, states0, dts, lams) -> None:
+        """lams[m] <- (dU^k / dU^0)^T lams[m] in place for every member, k = len(dts[m])."""
+        d = _dt_table(dts, self.members)
+        torch.cuda.synchronize()
+        check(self._L.chimera_badj_segment_vjp(self._h, _ptrs_many(states0),
+                                               d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
+                                               _ptrs_many(lams)), "chimera_badj_segment_vjp")
+
+    def set_tiles(self, shape_a: int = -1, shape_b: int = -1) -> None:
+        """Tile shapes of the two reverse passes: 0 = 16x16, 1 = 32x8."""
+        check(self._L.chimera_badj_set_tiles(self._h, int(shape_a), int(shape_b)),
+              "chimera_badj_set_tiles")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.chimera_badj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class _EvolveBatch(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, badj, dts, K, *U0s):
+        states = [u.detach().contiguous().clone() for u in U0s]
+        n = dts.shape[1]
+        ckpts = [[s.clone() for s in states]]
+        for s0 in range(0, n, K):
+            k = min(K, n - s0)
+            badj.forward(states, dts[:, s0:s0 + k], out=states)
+            if s0 + k < n:
+                ckpts.append([s.clone() for s in states])
+        ctx.ckpts, ctx.badj, ctx.dts, ctx.K = ckpts, badj, dts, K
+        return tuple(states)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        ckpts, badj, dts, K = ctx.ckpts, ctx.badj, ctx.dts, ctx.K
+        n = dts.shape[1]
+        lams = [g.contiguous().clone() for g in grads]
+        for seg in reversed(range(len(ckpts))):
+            s0 = seg * K
+            s1 = min(s0 + K, n)
+            badj.segment_vjp(ckpts[seg], np.ascontiguousarray(dts[:, s0:s1]), lams)
+        return (None, None, None, *lams)
+
+
+def evolve_batch(U0s, dts, badj: BatchAdjoint, checkpoint_every: int = 50):
+    """U_N of every member after dts.shape[1] steps (member m with the dts dts[m]),
+    through the batched forward and adjoint; differentiable with respect to every U0
+    (float64 CUDA tensors (6, ny, nx) of the sizes badj was made for)."""
+    d = np.ascontiguousarray(dts, dtype=np.float64)
+    assert d.shape[0] == len(U0s) == badj.members
+    return _EvolveBatch.apply(badj, d, max(1, int(checkpoint_every)), *U0s)
+
+
 # ---- differentiable set-up and diagnostics (torch ops) -------------------------
 
 def curl_2d_periodic(Az: torch.Tensor, dx: float):

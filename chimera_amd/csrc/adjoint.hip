@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "adjoint_phys.hpp"
+#include "adjoint_tile.hpp"
 #include "common.hpp"
 #include "resident.hpp"
 #include "riemann_adj.hpp"
@@ -24,12 +25,6 @@ namespace chimera {
 namespace {
 
 constexpr int BS = 256;
-
-struct AdjGrid {
-    int nx, ny, px, py;
-    double dx, vol, gamma;
-    int use_lim, mood;
-};
 
 using StageBufs = StageStore;
 
@@ -169,36 +164,6 @@ __global__ void ar_corner(AdjGrid G, RevBufs R, double dt) {
     }
 }
 
-// Reverse of the thermal check, the rho/P face floor and MOOD for one face state of
-// cell k (side s): lam of the face -> lam of the MUSCL candidate (cand) or of the
-// cell value (lreset).
-__device__ __forceinline__ void face_state_reverse(double lam[6], int s, bool to_cell,
-                                                   unsigned long long bits, double lreset[6],
-                                                   double *const cand[6], size_t k) {
-    if (bits & (1ull << (24 + s))) {   // thermal reset: every field took the cell value
-#pragma unroll
-        for (int f = 0; f < 6; ++f) {
-            lreset[f] += lam[f];
-            lam[f] = 0.0;
-        }
-    }
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        double v = lam[f];
-        if (f == 0 || f == 3) {
-            const int kk = f == 0 ? 0 : 1;
-            if (bits & (1ull << (32 + 4 * kk + s))) v = 0.0;   // floored to 1e-8
-        }
-        const bool reset = (bits >> (4 * f + s)) & 1ull;
-        if (reset || to_cell) {
-            lreset[f] += v;
-            cand[f][k] = 0.0;
-        } else {
-            cand[f][k] = v;
-        }
-    }
-}
-
 // Per face owner cell, one direction: lam of the face flux (from the update of the
 // two cells beside it and from the corner EMFs), the Riemann VJP, then the reverse
 // of the face-state construction.  XDIR: face (i+1/2, j); else (i, j+1/2), rotated.
@@ -247,8 +212,9 @@ __device__ __forceinline__ void face_reverse(const AdjGrid &G, const StageBufs &
     const bool lastf = XDIR ? (!G.px && i == nx - 1) : (!G.py && j == ny - 1);
     double *const *candL = XDIR ? R.lXL : R.lYL;
     double *const *candR = XDIR ? R.lXR : R.lYR;
-    face_state_reverse(lL, sL, false, bits, lreset, candL, k);
-    face_state_reverse(lR, sR, lastf, bits, lreset, candR, k);
+    const size_t n = (size_t)nx * ny;   // the reverse arrays are contiguous, n apart
+    face_state_reverse(lL, sL, false, bits, lreset, candL[0], n, k);
+    face_state_reverse(lR, sR, lastf, bits, lreset, candR[0], n, k);
 }
 
 template <int SOLVER>
@@ -363,600 +329,31 @@ __global__ void ar_bavg(AdjGrid G, RevBufs R, double *const *lin) {
     }
 }
 
-// ---- fused reverse -----------------------------------------------------------------
-//
-// The same reverse in two LDS-tiled kernels that recompute the forward stage instead of
-// reading stored intermediates.
-//
-// ar_tile_a (one TX x TY tile of owned cells per workgroup) replaces the STORE
-// recompute, ar_cell_tail, ar_corner and ar_faces: it loads the primitives of cells
-// [-2, TX+2] x [-2, TY+2], computes the fluxes of the faces of cells [0, TX] x [0, TY]
-// (forward phase 2), the flux update, floors, outflow correction and blend of those
-// cells with the reverse of the tail (lam of Up into LDS), lam of -Ez at the corners
-// around the owned faces, and finally, per owned cell, the face-state recompute, the
-// Riemann VJP and the reverse of the face construction of its x and y faces.  It
-// writes what the slope / primitive reverse needs (stage primitives, lam of the
-// face-state candidates, lam of the cell resets, of the outflow primitives and of the
-// cell B) and the update's and CT's part of lam_in; lam_u0 is accumulated.
-//
-// ar_tile_b replaces ar_slopes, ar_prims and ar_bavg: field by field it gathers the
-// slope cotangents of cells [-1, TX+1] x [-1, TY+1] into LDS, sums each cell's own and
-// its neighbours' parts for cells [0, TX] x [0, TY], then reverses cons_to_prim and the
-// face -> cell B average.  Every value is computed with the arithmetic, order and
-// branch of the unfused kernels, so both paths give the same bits.
-
-template <int TX, int TY>
-struct RevShapeA {
-    static constexpr int PW = TX + 5, PH = TY + 5, P_SZ = PW * PH;        // cells [-2, T+2]
-    static constexpr int FXW = TX + 2, FXH = TY + 1, FX_SZ = FXW * FXH;   // x faces: left cell [-1, TX] x [0, TY]
-    static constexpr int FYW = TX + 1, FYH = TY + 2, FY_SZ = FYW * FYH;   // y faces: lower cell [0, TX] x [-1, TY]
-    static constexpr int CW = TX + 1, CH = TY + 1, C_SZ = CW * CH;        // lam Up: cells [0, TX] x [0, TY]
-    static constexpr int EW = TX + 1, EH = TY + 1, E_SZ = EW * EH;        // lam -Ez: corners [-1, TX) x [-1, TY)
-    static constexpr int LDS_DOUBLES = 6 * P_SZ + 4 * FX_SZ + 4 * FY_SZ + 4 * C_SZ + E_SZ;
-};
-
-// Face states of one face from the primitives in LDS exactly as the forward stage
-// builds them (slopes of both cells, MUSCL, MOOD, rho/P floor, thermal check).  b0
-// points at the left / lower cell's rho; sA / sT are the strides along / across the
-// face normal; returns the MOOD and thermal bits and the floor bits at the positions
-// of the stored words (direction dir).
-template <int LIMITER>
-__device__ __forceinline__ unsigned long long face_pair(const double *b0, int psz, int sA, int sT,
-                                                        bool last, bool mood, bool use_lim,
-                                                        double inv_dx, double inv_2dx, double hdx,
-                                                        int dir, double L[6], double R[6]) {
-    double C[6];
-    unsigned bits = 0, fbits = 0;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const double *b = b0 + q * psz;
-        const double m = b[-sA], c = b[0], p = b[sA], pp = b[2 * sA];
-        C[q] = c;
-        const double sl = cell_slope<LIMITER>(m, c, p, use_lim, inv_dx, inv_2dx);
-        const double sr = cell_slope<LIMITER>(c, p, pp, use_lim, inv_dx, inv_2dx);
-        unsigned fb;
-        bits |= face_states(m, c, p, b[sT], b[-sT], sl, sr, last, mood, q == F_RHO || q == F_P,
-                            hdx, L[q], R[q], fb)
-                << (4 * q + 2 * dir);
-        if (q == F_RHO) fbits |= fb << (2 * dir);
-        if (q == F_P) fbits |= fb << (4 + 2 * dir);
-    }
-    if (thermal_reset(L, C)) bits |= 1u << (24 + 2 * dir);
-    if (thermal_reset(R, C)) bits |= 1u << (25 + 2 * dir);
-    return (unsigned long long)bits | ((unsigned long long)fbits << 32);
-}
+// ---- fused reverse kernels (one tile per workgroup, XCD-aware tile map) --------------
 
 template <int SOLVER, int LIMITER, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, StageBufs S, RevBufs R,
+__global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, FusedBufs F,
                                                    const double *const *lout, double *const *lin,
                                                    double *const *lU0, int stage, double dt,
                                                    int tiles_x, int ntiles) {
-    using A = RevShapeA<TX, TY>;
-    constexpr int NT = TX * TY;
-    __shared__ double lds[A::LDS_DOUBLES];
+    __shared__ double lds[RevShapeA<TX, TY>::LDS_DOUBLES];
     const int b = blockIdx.x, nb = gridDim.x;   // nb is a multiple of 8
     const int tile = (b % 8) * (nb / 8) + b / 8;
     if (tile >= ntiles) return;
-    const int tid = threadIdx.x;
-    const int nx = g.nx, ny = g.ny, px = g.px, py = g.py;
-    const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
-    double *sP = lds;                          // [6][PH][PW]
-    double *sFX = sP + 6 * A::P_SZ;            // [4][FXH][FXW] mass, Momx, Momy, energy
-    double *sFY = sFX + 4 * A::FX_SZ;          // [4][FYH][FYW]
-    double *sLU = sFY + 4 * A::FY_SZ;          // [4][CH][CW]
-    double *sLE = sLU + 4 * A::C_SZ;           // [EH][EW]
-    const double dx = g.dx, vol = g.vol, gam = g.gamma;
-    const double inv_dx = 1.0 / dx, inv_2dx = 1.0 / (2.0 * dx), hdx = 0.5 * dx;
-    const bool use_lim = G.use_lim != 0, mood = G.mood != 0;
-
-    // loads of phase 3 (the stage input, u^n and lam_out of cells [0, TX] x [0, TY]),
-    // issued first so that they arrive while phases 1 and 2 compute
-    constexpr int CPT = (A::C_SZ + NT - 1) / NT;
-    const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
-    const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
-    size_t ck[CPT];
-    bool cvalid[CPT];
-    double cin[CPT][4], cu0[CPT][4], clo[CPT][6];
-#pragma unroll
-    for (int u = 0; u < CPT; ++u) {
-        const int c = tid + u * NT;
-        const int gx = x0 + c % A::CW, gy = y0 + c / A::CW;
-        cvalid[u] = c < A::C_SZ && !((!px && gx >= nx) || (!py && gy >= ny));
-        ck[u] = cvalid[u] ? (size_t)map_index(gx, nx, px) + (size_t)nx * map_index(gy, ny, py) : 0;
-        if (cvalid[u]) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                cin[u][q] = g.uin[q][ck[u]];
-                cu0[u][q] = stage > 1 ? g.u0[q][ck[u]] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < 6; ++q) clo[u][q] = lout[q][ck[u]];
-        }
-    }
-
-    // ---- 1: primitives of cells [-2, TX+2] x [-2, TY+2] ------------------------------
-    for (int idx = tid; idx < A::P_SZ; idx += NT) {
-        const int lx = idx % A::PW, ly = idx / A::PW;
-        const int gi = map_index(x0 - 2 + lx, nx, px);
-        const int gj = map_index(y0 - 2 + ly, ny, py);
-        const int gim = gi > 0 ? gi - 1 : (px ? nx - 1 : 0);
-        const int gjm = gj > 0 ? gj - 1 : (py ? ny - 1 : 0);
-        const size_t k = (size_t)gi + (size_t)nx * gj;
-        const double Bx = 0.5 * (g.uin[4][k] + g.uin[4][(size_t)gim + (size_t)nx * gj]);
-        const double By = 0.5 * (g.uin[5][k] + g.uin[5][(size_t)gi + (size_t)nx * gjm]);
-        double rho, vx, vy, P;
-        cons_to_prim(g.uin[0][k], g.uin[1][k], g.uin[2][k], g.uin[3][k], Bx, By, gam, vol, rho, vx,
-                     vy, P);
-        sP[0 * A::P_SZ + idx] = rho;
-        sP[1 * A::P_SZ + idx] = vx;
-        sP[2 * A::P_SZ + idx] = vy;
-        sP[3 * A::P_SZ + idx] = P;
-        sP[4 * A::P_SZ + idx] = Bx;
-        sP[5 * A::P_SZ + idx] = By;
-        const int ci = lx - 2, cj = ly - 2;
-        if (ci >= 0 && ci < TX && cj >= 0 && cj < TY && x0 + ci < nx && y0 + cj < ny) {
-            const size_t ko = (size_t)(x0 + ci) + (size_t)nx * (y0 + cj);
-            S.P[0][ko] = rho;
-            S.P[1][ko] = vx;
-            S.P[2][ko] = vy;
-            S.P[3][ko] = P;
-            S.P[4][ko] = Bx;
-            S.P[5][ko] = By;
-        }
-    }
-    __syncthreads();
-
-    // ---- 2: fluxes of the faces of cells [0, TX] x [0, TY] ---------------------------
-    for (int f = tid; f < A::FX_SZ + A::FY_SZ; f += NT) {
-        const int dir = f >= A::FX_SZ;
-        int i, j;
-        if (!dir) {
-            i = f % A::FXW - 1;
-            j = f / A::FXW;
-        } else {
-            i = (f - A::FX_SZ) % A::FYW;
-            j = (f - A::FX_SZ) / A::FYW - 1;
-        }
-        const int lpos = (j + 2) * A::PW + (i + 2);
-        const bool last = dir ? (!py && map_index(y0 + j, ny, py) == ny - 1)
-                              : (!px && map_index(x0 + i, nx, px) == nx - 1);
-        double L[6], Rs[6];
-        (void)face_pair<LIMITER>(sP + lpos, A::P_SZ, dir ? A::PW : 1, dir ? 1 : A::PW, last, mood,
-                                 use_lim, inv_dx, inv_2dx, hdx, dir, L, Rs);
-        FaceState fa{L[F_RHO], dir ? L[F_VY] : L[F_VX], dir ? L[F_VX] : L[F_VY], L[F_P],
-                     dir ? L[F_BY] : L[F_BX], dir ? L[F_BX] : L[F_BY]};
-        FaceState fb{Rs[F_RHO], dir ? Rs[F_VY] : Rs[F_VX], dir ? Rs[F_VX] : Rs[F_VY], Rs[F_P],
-                     dir ? Rs[F_BY] : Rs[F_BX], dir ? Rs[F_BX] : Rs[F_BY]};
-        const Flux5 fl = riemann<SOLVER>(fa, fb, gam);
-        double *sF = dir ? sFY : sFX;
-        const int fsz = dir ? A::FY_SZ : A::FX_SZ;
-        const int idx = dir ? f - A::FX_SZ : f;
-        sF[0 * fsz + idx] = fl.mass;
-        sF[1 * fsz + idx] = dir ? fl.mom_t : fl.mom_n;
-        sF[2 * fsz + idx] = dir ? fl.mom_n : fl.mom_t;
-        sF[3 * fsz + idx] = fl.energy;
-    }
-    __syncthreads();
-
-    // ---- 3: forward tail and its reverse for cells [0, TX] x [0, TY] ------------------
-    const double dtdx = dt * dx;
-#pragma unroll
-    for (int u = 0; u < CPT; ++u) {
-        const int c = tid + u * NT;
-        if (c >= A::C_SZ) break;
-        const int ci = c % A::CW, cj = c / A::CW;
-        const int gx = x0 + ci, gy = y0 + cj;
-        if (!cvalid[u]) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) sLU[q * A::C_SZ + c] = 0.0;
-            continue;
-        }
-        const int gi = map_index(gx, nx, px), gj = map_index(gy, ny, py);
-        const size_t k = ck[u];
-        const int xr = cj * A::FXW + (ci + 1);         // x face right of the cell
-        const int yr = (cj + 1) * A::FYW + ci;         // y face above the cell
-        const bool xlo_open = (gi == 0) && !px;
-        const bool ylo_open = (gj == 0) && !py;
-        double Up[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            double v = cin[u][q];
-            const double fr = sFX[q * A::FX_SZ + xr];
-            if (xlo_open) v = v - dtdx * fr;
-            else v = v - dtdx * fr + dtdx * sFX[q * A::FX_SZ + xr - 1];
-            const double ft = sFY[q * A::FY_SZ + yr];
-            if (ylo_open) v = v - dtdx * ft;
-            else v = v - dtdx * ft + dtdx * sFY[q * A::FY_SZ + yr - A::FYW];
-            Up[q] = v;
-        }
-        const int pc = (cj + 2) * A::PW + (ci + 2);
-        double P6[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) P6[q] = sP[q * A::P_SZ + pc];
-        const double Bx = P6[4], By = P6[5];
-        double U3[4] = {Up[0], Up[1], Up[2], Up[3]};
-        conserved_floors(U3[0], U3[1], U3[2], U3[3], Bx, By, vol, gam);
-        if (xlo_open || ylo_open) {
-            const double rho = P6[0], vx = P6[1], vy = P6[2], P = P6[3];
-            const double h = 0.5 * (Bx * Bx + By * By);
-            const double en = (P - h) / (gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy) + h;
-            if (xlo_open) {
-                U3[0] = U3[0] + dtdx * rho * vx;
-                U3[1] = U3[1] + dtdx * (rho * (vx * vx) + P - Bx * Bx);
-                U3[2] = U3[2] + dtdx * (rho * vx * vy - Bx * By);
-                U3[3] = U3[3] + dtdx * ((en + P) * vx - Bx * (Bx * vx + By * vy));
-            }
-            if (ylo_open) {
-                U3[0] = U3[0] + dtdx * rho * vy;
-                U3[2] = U3[2] + dtdx * (rho * (vy * vy) + P - By * By);
-                U3[1] = U3[1] + dtdx * (rho * vy * vx - By * Bx);
-                U3[3] = U3[3] + dtdx * ((en + P) * vy - By * (By * vy + Bx * vx));
-            }
-        }
-        // reverse (as ar_cell_tail)
-        const bool owned = ci < TX && cj < TY && gx < nx && gy < ny;
-        double lBx = 0.0, lBy = 0.0;
-        double l3[4], lb[2];
-        if (stage > 1) {
-            double V[4], lo[4], lV[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                V[q] = c0 * cu0[u][q] + c1 * U3[q];
-                lo[q] = clo[u][q];
-            }
-            floors_vjp(V, Bx, By, vol, gam, lo, lV, lBx, lBy);
-            // stage 3 starts lam_u0 (written), stage 2 adds to it
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (owned) {
-                    if (stage == 3) lU0[q][k] = c0 * lV[q];
-                    else lU0[q][k] += c0 * lV[q];
-                }
-                l3[q] = c1 * lV[q];
-            }
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                if (owned) {
-                    if (stage == 3) lU0[4 + q][k] = c0 * clo[u][4 + q];
-                    else lU0[4 + q][k] += c0 * clo[u][4 + q];
-                }
-                lb[q] = c1 * clo[u][4 + q];
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) l3[q] = clo[u][q];
-#pragma unroll
-            for (int q = 0; q < 2; ++q) lb[q] = clo[u][4 + q];
-        }
-        double lPd[6] = {0, 0, 0, 0, 0, 0};
-        if (xlo_open) outflow_vjp(0, P6, gam, dtdx, l3, lPd);
-        if (ylo_open) outflow_vjp(1, P6, gam, dtdx, l3, lPd);
-        double lUp[4];
-        floors_vjp(Up, Bx, By, vol, gam, l3, lUp, lBx, lBy);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sLU[q * A::C_SZ + c] = lUp[q];
-        if (owned) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) lin[q][k] = lUp[q];
-            lin[4][k] = lb[0];
-            lin[5][k] = lb[1];
-            R.lBc[0][k] = lBx;
-            R.lBc[1][k] = lBy;
-#pragma unroll
-            for (int f = 0; f < 6; ++f) R.lPd[f][k] = lPd[f];
-        }
-    }
-
-    // ---- 4: lam of -Ez at corners [-1, TX) x [-1, TY) (as ar_corner) -----------------
-    {
-        const double s = dt / dx;
-        const double cb = stage > 1 ? c1 : 1.0;   // lam of the CT-updated face B
-        const double *lox = lout[4], *loy = lout[5];
-        for (int e = tid; e < A::E_SZ; e += NT) {
-            const int la = e % A::EW - 1, lb_ = e / A::EW - 1;
-            const int gx = x0 + la, gy = y0 + lb_;
-            if ((!px && (gx < 0 || gx >= nx)) || (!py && (gy < 0 || gy >= ny))) {
-                sLE[e] = 0.0;
-                continue;
-            }
-            const int a = map_index(gx, nx, px), bb = map_index(gy, ny, py);
-            const size_t k = (size_t)a + (size_t)nx * bb;
-            double acc = 0.0;
-            if (bb >= 1 || py) acc += s * (cb * lox[k]);
-            {
-                const int jn = bb + 1;
-                if (jn < ny) acc -= s * (cb * lox[k + nx]);
-                else if (py) acc -= s * (cb * lox[(size_t)a]);
-            }
-            if (!py) {
-                if (bb == 1) acc += s * (cb * lox[(size_t)a]);
-                if (bb == 0) acc -= s * (cb * lox[(size_t)a]);
-            }
-            if (a >= 1 || px) acc -= s * (cb * loy[k]);
-            {
-                const int in = a + 1;
-                if (in < nx) acc += s * (cb * loy[k + 1]);
-                else if (px) acc += s * (cb * loy[(size_t)nx * bb]);
-            }
-            if (!px) {
-                if (a == 1) acc -= s * (cb * loy[(size_t)nx * bb]);
-                if (a == 0) acc += s * (cb * loy[(size_t)nx * bb]);
-            }
-            sLE[e] = acc;
-        }
-    }
-    __syncthreads();
-
-    // ---- 5: owned faces (as ar_faces): per owned cell its x face, then its y face ------
-    {
-        const int ci = tid % TX, cj = tid / TX;
-        const int gi = x0 + ci, gj = y0 + cj;
-        const bool active = gi < nx && gj < ny;
-        const size_t k = active ? (size_t)gi + (size_t)nx * gj : 0;
-        // lam of the two face states of the face dir of this cell, and its bits
-        auto face_rev = [&](int dir, double lL[6], double lR[6]) -> unsigned long long {
-            auto lEz = [&](int a, int bb) { return -sLE[(bb + 1) * A::EW + (a + 1)]; };
-            const double *b0 = sP + (cj + 2) * A::PW + (ci + 2);
-            const bool lastf = dir ? (!py && gj == ny - 1) : (!px && gi == nx - 1);
-            double L[6], Rr[6];
-            const unsigned long long bits =
-                face_pair<LIMITER>(b0, A::P_SZ, dir ? A::PW : 1, dir ? 1 : A::PW, lastf, mood,
-                                   use_lim, inv_dx, inv_2dx, hdx, dir, L, Rr);
-            const bool has_next = dir ? (gj < ny - 1 || py) : (gi < nx - 1 || px);
-            const int cn = dir ? (cj + 1) * A::CW + ci : cj * A::CW + ci + 1;
-            const int cc = cj * A::CW + ci;
-            double lF[5];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                double v = -dtdx * sLU[q * A::C_SZ + cc];
-                if (has_next) v += dtdx * sLU[q * A::C_SZ + cn];
-                lF[q] = v;
-            }
-            if (!dir) {
-                double v = -0.25 * lEz(ci, cj);
-                if (gj >= 1 || py) v += -0.25 * lEz(ci, cj - 1);
-                if (!py && gj == ny - 1) v += -0.25 * lEz(ci, cj);
-                lF[4] = v;
-            } else {
-                double v = 0.25 * lEz(ci, cj);
-                if (gi >= 1 || px) v += 0.25 * lEz(ci - 1, cj);
-                if (!px && gi == nx - 1) v += 0.25 * lEz(ci, cj);
-                lF[4] = v;
-            }
-            const int fu = dir ? 2 : 1, fw = dir ? 1 : 2, fbn = dir ? 5 : 4, fbt = dir ? 4 : 5;
-            FaceState fa{L[0], L[fu], L[fw], L[3], L[fbn], L[fbt]};
-            FaceState fb{Rr[0], Rr[fu], Rr[fw], Rr[3], Rr[fbn], Rr[fbt]};
-            double lFr[5] = {lF[0], dir ? lF[2] : lF[1], dir ? lF[1] : lF[2], lF[3], lF[4]};
-            double lLr[6], lRr[6];
-            riemann_rev<SOLVER>(fa, fb, gam, lFr, lLr, lRr);
-            lL[0] = lLr[0]; lL[fu] = lLr[1]; lL[fw] = lLr[2]; lL[3] = lLr[3]; lL[fbn] = lLr[4]; lL[fbt] = lLr[5];
-            lR[0] = lRr[0]; lR[fu] = lRr[1]; lR[fw] = lRr[2]; lR[3] = lRr[3]; lR[fbn] = lRr[4]; lR[fbt] = lRr[5];
-            return bits;
-        };
-        if (active) {
-            double lreset[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int dir = 0; dir < 2; ++dir) {
-                double lL[6], lR[6];
-                const unsigned long long bits = face_rev(dir, lL, lR);
-                const bool lastf = dir ? (!py && gj == ny - 1) : (!px && gi == nx - 1);
-                face_state_reverse(lL, 2 * dir, false, bits, lreset, dir ? R.lYL : R.lXL, k);
-                face_state_reverse(lR, 2 * dir + 1, lastf, bits, lreset, dir ? R.lYR : R.lXR, k);
-            }
-#pragma unroll
-            for (int f = 0; f < 6; ++f) R.lPr[f][k] = lreset[f];
-        }
-    }
+    rev_tile_a<SOLVER, LIMITER, TX, TY>(g, G, F, lout, lin, lU0, stage, dt, tile % tiles_x,
+                                        tile / tiles_x, lds);
 }
 
-template <int TX, int TY>
-struct RevShapeB {
-    static constexpr int PW = TX + 5, PH = TY + 5, P_SZ = PW * PH;   // primitive: cells [-2, T+2]
-    static constexpr int GW = TX + 3, GH = TY + 3, G_SZ = GW * GH;   // slope parts: cells [-1, T+1]
-    static constexpr int CW = TX + 1, CH = TY + 1, C_SZ = CW * CH;   // cells [0, T]
-    static constexpr int LDS_DOUBLES = P_SZ + 5 * G_SZ + 2 * C_SZ;
-    static constexpr int CPT = (C_SZ + TX * TY - 1) / (TX * TY);     // cells [0, T] per thread
-};
-
-// lU0_add: the step's lam_u0, added to lam_in at the end (stage 1), else nullptr
 template <int LIMITER, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void ar_tile_b(AdjGrid G, const double *const *U, StageStore S,
-                                                   RevBufs R, double *const *lin,
+__global__ __launch_bounds__(TX *TY) void ar_tile_b(AdjGrid G, const double *const *U, FusedBufs F,
+                                                   double *const *lin,
                                                    const double *const *lU0_add, int tiles_x,
                                                    int ntiles) {
-    using B = RevShapeB<TX, TY>;
-    constexpr int NT = TX * TY;
-    __shared__ double lds[B::LDS_DOUBLES];
+    __shared__ double lds[RevShapeB<TX, TY>::LDS_DOUBLES];
     const int b = blockIdx.x, nb = gridDim.x;
     const int tile = (b % 8) * (nb / 8) + b / 8;
     if (tile >= ntiles) return;
-    const int tid = threadIdx.x;
-    const int nx = G.nx, ny = G.ny, px = G.px, py = G.py;
-    const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
-    double *sA = lds;                       // one primitive, cells [-2, T+2]
-    double *sG = sA + B::P_SZ;              // gmx, gpx, gmy, gpy, lP of one field, cells [-1, T+1]
-    double *sB = sG + 5 * B::G_SZ;          // lam of cell B (total), cells [0, T]
-    const double hdx = 0.5 * G.dx, inv_dx = 1.0 / G.dx, inv_2dx = 1.0 / (2.0 * G.dx);
-    // valid global cell of a local position, or -1 outside a non-periodic grid
-    auto gcell = [&](int lx, int ly, int &gi, int &gj) -> bool {
-        const int ux = x0 + lx, uy = y0 + ly;
-        if ((!px && (ux < 0 || ux >= nx)) || (!py && (uy < 0 || uy >= ny))) return false;
-        gi = map_index(ux, nx, px);
-        gj = map_index(uy, ny, py);
-        return true;
-    };
-    double lp6[B::CPT][6];
-
-    // per-thread positions of the primitive items and of the slope items, fixed over the
-    // six fields; each field's loads are issued one round ahead of its use
-    constexpr int PPT = (B::P_SZ + NT - 1) / NT, GPT = (B::G_SZ + NT - 1) / NT;
-    size_t kp[PPT];
-#pragma unroll
-    for (int u = 0; u < PPT; ++u) {
-        const int idx = tid + u * NT;
-        const int gi = map_index(x0 - 2 + idx % B::PW, nx, px);
-        const int gj = map_index(y0 - 2 + idx / B::PW, ny, py);
-        kp[u] = idx < B::P_SZ ? (size_t)gi + (size_t)nx * gj : 0;
-    }
-    struct GItem {
-        size_t k, kl, kb;
-        bool valid, hl, hb, cxm, cxp, cym, cyp;
-    };
-    GItem gt[GPT];
-#pragma unroll
-    for (int u = 0; u < GPT; ++u) {
-        const int idx = tid + u * NT;
-        GItem &t = gt[u];
-        int i = 0, j = 0;
-        t.valid = idx < B::G_SZ && gcell(idx % B::GW - 1, idx / B::GW - 1, i, j);
-        t.k = (size_t)i + (size_t)nx * j;
-        t.hl = i > 0 || px;
-        t.hb = j > 0 || py;
-        t.kl = i > 0 ? t.k - 1 : t.k + nx - 1;
-        t.kb = j > 0 ? t.k - nx : t.k + (size_t)nx * (ny - 1);
-        t.cxm = !px && i == 0;   // clamped neighbours
-        t.cxp = !px && i == nx - 1;
-        t.cym = !py && j == 0;
-        t.cyp = !py && j == ny - 1;
-    }
-    double pv[2][PPT], rv[2][GPT][6];
-    auto load = [&](int f, int buf) {
-#pragma unroll
-        for (int u = 0; u < PPT; ++u)
-            if (tid + u * NT < B::P_SZ) pv[buf][u] = S.P[f][kp[u]];
-#pragma unroll
-        for (int u = 0; u < GPT; ++u) {
-            const GItem &t = gt[u];
-            if (!t.valid) continue;
-            rv[buf][u][0] = R.lXL[f][t.k];
-            rv[buf][u][1] = t.hl ? R.lXR[f][t.kl] : 0.0;   // right state of face i-1/2
-            rv[buf][u][2] = R.lYL[f][t.k];
-            rv[buf][u][3] = t.hb ? R.lYR[f][t.kb] : 0.0;
-            rv[buf][u][4] = R.lPr[f][t.k];
-            rv[buf][u][5] = R.lPd[f][t.k];
-        }
-    };
-    load(0, 0);
-
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        const int cur = f & 1;
-        // the field's stage primitive over cells [-2, T+2] (clamped like the forward)
-#pragma unroll
-        for (int u = 0; u < PPT; ++u)
-            if (tid + u * NT < B::P_SZ) sA[tid + u * NT] = pv[cur][u];
-        __syncthreads();
-        if (f < 5) load(f + 1, cur ^ 1);
-        // slope parts of cells [-1, T+1] (as ar_slopes)
-#pragma unroll
-        for (int u = 0; u < GPT; ++u) {
-            const int idx = tid + u * NT;
-            if (idx >= B::G_SZ) break;
-            const GItem &t = gt[u];
-            if (!t.valid) {
-#pragma unroll
-                for (int r = 0; r < 5; ++r) sG[r * B::G_SZ + idx] = 0.0;
-                continue;
-            }
-            const int lx = idx % B::GW - 1, ly = idx / B::GW - 1;
-            const double *a = sA + (ly + 2) * B::PW + (lx + 2);
-            const double am = t.cxm ? a[0] : a[-1], ap = t.cxp ? a[0] : a[1];
-            const double bm = t.cym ? a[0] : a[-B::PW], bp = t.cyp ? a[0] : a[B::PW];
-            const double *r6 = rv[cur][u];
-            const double xr = r6[1], yr = r6[3];
-            const double lSx = hdx * (r6[0] - xr);
-            const double lSy = hdx * (r6[2] - yr);
-            double lp = r6[0] + xr + r6[2] + yr + r6[4] + r6[5];
-            double dm, d0, dp;
-            if (G.use_lim) limited_slope_grad<LIMITER>(am, a[0], ap, inv_dx, dm, d0, dp);
-            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
-            double gm = lSx * dm, gp = lSx * dp;
-            lp += lSx * d0;
-            if (t.cxm) { lp += gm; gm = 0.0; }
-            if (t.cxp) { lp += gp; gp = 0.0; }
-            sG[0 * B::G_SZ + idx] = gm;
-            sG[1 * B::G_SZ + idx] = gp;
-            if (G.use_lim) limited_slope_grad<LIMITER>(bm, a[0], bp, inv_dx, dm, d0, dp);
-            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
-            gm = lSy * dm;
-            gp = lSy * dp;
-            lp += lSy * d0;
-            if (t.cym) { lp += gm; gm = 0.0; }
-            if (t.cyp) { lp += gp; gp = 0.0; }
-            sG[2 * B::G_SZ + idx] = gm;
-            sG[3 * B::G_SZ + idx] = gp;
-            sG[4 * B::G_SZ + idx] = lp;
-        }
-        __syncthreads();
-        // own part plus the neighbours' parts for cells [0, T] (as ar_prims)
-#pragma unroll
-        for (int u = 0; u < B::CPT; ++u) {
-            const int c = tid + u * NT;
-            if (c >= B::C_SZ) break;
-            const int lx = c % B::CW, ly = c / B::CW;
-            int i, j;
-            if (!gcell(lx, ly, i, j)) {
-                lp6[u][f] = 0.0;
-                continue;
-            }
-            const bool hl = i > 0 || px, hr = i < nx - 1 || px;
-            const bool hb = j > 0 || py, ht = j < ny - 1 || py;
-            const int gidx = (ly + 1) * B::GW + (lx + 1);
-            double acc = sG[4 * B::G_SZ + gidx];
-            if (hl) acc += sG[1 * B::G_SZ + gidx - 1];
-            if (hr) acc += sG[0 * B::G_SZ + gidx + 1];
-            if (hb) acc += sG[3 * B::G_SZ + gidx - B::GW];
-            if (ht) acc += sG[2 * B::G_SZ + gidx + B::GW];
-            lp6[u][f] = acc;
-        }
-        __syncthreads();
-    }
-    // reverse of cons_to_prim for cells [0, T]; lam of the conserved input for owned cells
-#pragma unroll
-    for (int u = 0; u < B::CPT; ++u) {
-        const int c = tid + u * NT;
-        if (c >= B::C_SZ) break;
-        const int lx = c % B::CW, ly = c / B::CW;
-        int i, j;
-        if (!gcell(lx, ly, i, j)) {
-            sB[c] = 0.0;
-            sB[B::C_SZ + c] = 0.0;
-            continue;
-        }
-        const size_t k = (size_t)i + (size_t)nx * j;
-        const double Bx = S.P[4][k], By = S.P[5][k];
-        double lU[4] = {0, 0, 0, 0};
-        double lBx = lp6[u][4] + R.lBc[0][k], lBy = lp6[u][5] + R.lBc[1][k];
-        cons_to_prim_vjp(U[0][k], U[1][k], U[2][k], U[3][k], Bx, By, G.gamma, G.vol, lp6[u], lU,
-                         lBx, lBy);
-        if (lx < TX && ly < TY && x0 + lx < nx && y0 + ly < ny) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) lin[q][k] += lU[q];
-        }
-        sB[c] = lBx;
-        sB[B::C_SZ + c] = lBy;
-    }
-    __syncthreads();
-    // face -> cell B average (as ar_bavg), owned cells
-    {
-        const int lx = tid % TX, ly = tid / TX;
-        const int i = x0 + lx, j = y0 + ly;
-        if (i < nx && j < ny) {
-            const size_t k = (size_t)i + (size_t)nx * j;
-            const int c = ly * B::CW + lx;
-            const double *lbx = sB, *lby = sB + B::C_SZ;
-            double ax = (i == 0 && !px) ? lbx[c] : 0.5 * lbx[c];
-            if (i + 1 < nx || px) ax += 0.5 * lbx[c + 1];
-            double ay = (j == 0 && !py) ? lby[c] : 0.5 * lby[c];
-            if (j + 1 < ny || py) ay += 0.5 * lby[c + B::CW];
-            lin[4][k] += ax;
-            lin[5][k] += ay;
-            if (lU0_add) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) lin[q][k] += lU0_add[q][k];
-            }
-        }
-    }
+    rev_tile_b<LIMITER, TX, TY>(G, U, F, lin, lU0_add, tile % tiles_x, tile / tiles_x, lds);
 }
 
 // Face VJPs of one solver for n faces (x direction), by hand-written reverse (method 0)
@@ -1010,6 +407,7 @@ struct Ptr6 {
 struct chimera_adj {
     AdjGrid G{};
     bool fused = true;                        // fused reverse (ar_tile_a / ar_tile_b)
+    FusedBufs F{nullptr, 0};                  // their exchange arrays
     int rshape_a = 3, rshape_b = 3;           // their tile shapes (REV_SHAPES)
     chimera_scheme_t sch{};
     hipStream_t stream = nullptr;
@@ -1116,9 +514,9 @@ int run_forward_stage(chimera_adj *a, int stage, const Ptr6 &in, const Ptr6 &u0,
     return launch_check("adjoint forward stage");
 }
 
-using TileAFn = void (*)(GridView, AdjGrid, StageBufs, RevBufs, const double *const *,
-                         double *const *, double *const *, int, double, int, int);
-using TileBFn = void (*)(AdjGrid, const double *const *, StageStore, RevBufs, double *const *,
+using TileAFn = void (*)(GridView, AdjGrid, FusedBufs, const double *const *, double *const *,
+                         double *const *, int, double, int, int);
+using TileBFn = void (*)(AdjGrid, const double *const *, FusedBufs, double *const *,
                          const double *const *, int, int);
 
 template <int TX, int TY>
@@ -1168,12 +566,12 @@ int run_reverse_stage_fused(chimera_adj *a, int stage, const Ptr6 &in, Tab tin, 
     rev_launch_shape(a, a->rshape_a, tx, nt, grid);
     const RevTile ta = REV_SHAPES[a->rshape_a];
     hipLaunchKernelGGL(tile_a_fn(a->sch.riemann_solver, a->sch.slope_limiter, a->rshape_a),
-                       dim3(grid), dim3(ta.tx * ta.ty), 0, a->stream, g, a->G, a->S[stage - 1],
-                       a->R, tlout, tlin, table(a, TB_LU0), stage, dt, tx, nt);
+                       dim3(grid), dim3(ta.tx * ta.ty), 0, a->stream, g, a->G, a->F, tlout, tlin,
+                       table(a, TB_LU0), stage, dt, tx, nt);
     rev_launch_shape(a, a->rshape_b, tx, nt, grid);
     const RevTile tb = REV_SHAPES[a->rshape_b];
     hipLaunchKernelGGL(tile_b_fn(a->sch.slope_limiter, a->rshape_b), dim3(grid),
-                       dim3(tb.tx * tb.ty), 0, a->stream, a->G, tin, a->S[stage - 1], a->R, tlin,
+                       dim3(tb.tx * tb.ty), 0, a->stream, a->G, tin, a->F, tlin,
                        stage == 1 ? (const double *const *)table(a, TB_LU0) : nullptr, tx, nt);
     return launch_check("adjoint fused reverse stage");
 }
@@ -1270,6 +668,7 @@ extern "C" int chimera_adj_destroy(chimera_adj_t *a) {
     if (!a) return CHIMERA_OK;
     if (a->stream) (void)hipStreamSynchronize(a->stream);
     (void)hipFree(a->arena);
+    (void)hipFree(a->F.base);
     (void)hipFree(a->dptr);
     (void)hipFree(a->dt_dev);
     (void)hipFree(a->traj);
@@ -1310,7 +709,9 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
     // reverse: 4 + 2 + 2 + 6 + 1 + 24 + 6 + 24 + 6 + 2 = 77; states 3 x 6; lambdas 5 x 6
     const size_t per_stage = 40, rev = 77, extra = 18 + 30;
     const size_t total = 3 * per_stage + rev + extra;
+    a->F.n = a->n;
     if (hipMalloc(&a->arena, total * a->n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&a->F.base, FusedBufs::ARRAYS * a->n * sizeof(double)) != hipSuccess ||
         hipMalloc(&a->dptr, 6 * TB_COUNT * sizeof(double *)) != hipSuccess) {
         set_error("adjoint workspace allocation (%zu arrays of %zu cells) failed", total, a->n);
         chimera_adj_destroy(a);

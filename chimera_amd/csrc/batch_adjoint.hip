@@ -1,0 +1,466 @@
+// Batched discrete adjoint of an ensemble of independent grids of uneven sizes (the
+// differentiable GRF ensemble): every forward and reverse stage is ONE persistent
+// launch over a tile table spanning all members, as the batched forward ensemble does.
+//
+// Per member: a prescribed dt sequence (frozen), a trajectory of every stage input of
+// the current segment, the fused-reverse exchange arrays and two lam buffers.  A
+// segment sweep recomputes the segment (3 stage launches per step over all members),
+// then per step, newest first, runs the fused reverse pair (rev_tile_a / rev_tile_b of
+// adjoint_tile.hpp) for stages 3, 2, 1 over all members; each member's lam is updated in
+// place.  The arithmetic per member is that of the single-grid adjoint, so a member's
+// gradient is bit-identical to chimera_adj_segment_vjp's.
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "adjoint_tile.hpp"
+#include "common.hpp"
+#include "resident.hpp"
+
+namespace chimera {
+namespace {
+
+struct BAdjMember {
+    AdjGrid G;
+    size_t n;
+    double *traj;               // 3 x cap slots x 6 fields x n
+    const double *state0[6];    // the caller's U^0 of the segment (device)
+    double *lam_io[6];          // the caller's lam (device, in / out)
+    double *state_out[6];       // forward: where the last step's output goes
+    double *lam[3][6];          // LA, LB, lam_u0
+    FusedBufs F;
+    const double *dts;          // this member's dts (device)
+};
+
+struct BTile {
+    int member, tx, ty;
+};
+
+// the block's share of the tile table: XCD b % 8 walks its contiguous eighth
+__device__ __forceinline__ void badj_segment(int ntiles, int &begin, int &end, int &slot,
+                                             int &slots) {
+    const int xcd = blockIdx.x & 7;
+    slot = blockIdx.x >> 3;
+    slots = gridDim.x >> 3;
+    begin = (int)((long long)ntiles * xcd / 8);
+    end = (int)((long long)ntiles * (xcd + 1) / 8);
+}
+
+__device__ __forceinline__ double *slot_field(const BAdjMember &m, int slot, int q) {
+    return m.traj + ((size_t)6 * slot + q) * m.n;
+}
+
+// stage input of step s, stage st (1..3): U^s, U^(1), U^(2)
+__device__ __forceinline__ const double *stage_in(const BAdjMember &m, int s, int st, int q) {
+    if (st == 1) return s == 0 ? m.state0[q] : slot_field(m, 3 * s, q);
+    return slot_field(m, 3 * s + st - 1, q);
+}
+
+__device__ __forceinline__ GridView member_view(const BAdjMember &m, int s, int st) {
+    GridView g;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        g.uin[q] = stage_in(m, s, st, q);
+        g.u0[q] = stage_in(m, s, 1, q);
+        g.uout[q] = nullptr;
+    }
+    g.dt = m.dts + s;
+    g.mask = nullptr;
+    g.nx = m.G.nx;
+    g.ny = m.G.ny;
+    g.px = m.G.px;
+    g.py = m.G.py;
+    g.dx = m.G.dx;
+    g.vol = m.G.vol;
+    g.gamma = m.G.gamma;
+    return g;
+}
+
+// Forward stage st of step s for every member; the output goes to the trajectory (or,
+// with to_out, to the member's state_out).
+template <int SOLVER, int LIMITER, int TX, int TY>
+__global__ __launch_bounds__(STAGE_THREADS) void badj_forward_kernel(
+    const BAdjMember *__restrict__ mem, const BTile *__restrict__ tiles, int ntiles, int s, int st,
+    int to_out, int use_lim, int mood) {
+    __shared__ double lds[TileShape<TX, TY>::LDS_DOUBLES];
+    int begin, end, slot, slots;
+    badj_segment(ntiles, begin, end, slot, slots);
+    for (int t = begin + slot; t < end; t += slots) {
+        const BTile tr = tiles[t];
+        const BAdjMember &m = mem[tr.member];
+        GridView g = member_view(m, s, st);
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            g.uout[q] = to_out ? m.state_out[q] : slot_field(m, 3 * s + st, q);
+        stage_tile<SOLVER, LIMITER, TX, TY>(g, st, use_lim != 0, mood != 0, tr.tx, tr.ty, lds);
+        __syncthreads();   // the next tile reuses LDS
+    }
+}
+
+// lam buffers of stage st: lout -> lin (stage 3: lam_io -> LB, 2: LB -> LA, 1: LA -> lam_io)
+__device__ __forceinline__ void stage_lams(const BAdjMember &m, int st, const double *lout[6],
+                                           double *lin[6]) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        lout[q] = st == 3 ? m.lam_io[q] : (st == 2 ? m.lam[1][q] : m.lam[0][q]);
+        lin[q] = st == 3 ? m.lam[1][q] : (st == 2 ? m.lam[0][q] : m.lam_io[q]);
+    }
+}
+
+template <int SOLVER, int LIMITER, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void badj_rev_a_kernel(const BAdjMember *__restrict__ mem,
+                                                           const BTile *__restrict__ tiles,
+                                                           int ntiles, int s, int st) {
+    __shared__ double lds[RevShapeA<TX, TY>::LDS_DOUBLES];
+    int begin, end, slot, slots;
+    badj_segment(ntiles, begin, end, slot, slots);
+    for (int t = begin + slot; t < end; t += slots) {
+        const BTile tr = tiles[t];
+        const BAdjMember &m = mem[tr.member];
+        const GridView g = member_view(m, s, st);
+        const double *lout[6];
+        double *lin[6], *lu0[6];
+        stage_lams(m, st, lout, lin);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) lu0[q] = m.lam[2][q];
+        double dt = m.dts[s];
+        rev_tile_a<SOLVER, LIMITER, TX, TY>(g, m.G, m.F, lout, lin, lu0, st, dt, tr.tx, tr.ty, lds);
+        __syncthreads();
+    }
+}
+
+template <int LIMITER, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void badj_rev_b_kernel(const BAdjMember *__restrict__ mem,
+                                                           const BTile *__restrict__ tiles,
+                                                           int ntiles, int s, int st) {
+    __shared__ double lds[RevShapeB<TX, TY>::LDS_DOUBLES];
+    int begin, end, slot, slots;
+    badj_segment(ntiles, begin, end, slot, slots);
+    for (int t = begin + slot; t < end; t += slots) {
+        const BTile tr = tiles[t];
+        const BAdjMember &m = mem[tr.member];
+        const double *U[6], *lout[6], *lu0[6];
+        double *lin[6];
+        stage_lams(m, st, lout, lin);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            U[q] = stage_in(m, s, st, q);
+            lu0[q] = m.lam[2][q];
+        }
+        rev_tile_b<LIMITER, TX, TY>(m.G, U, m.F, lin, st == 1 ? lu0 : nullptr, tr.tx, tr.ty, lds);
+        __syncthreads();
+    }
+}
+
+// tile shapes: forward 16 x 16; reverse 0 = 16 x 16, 1 = 32 x 8
+constexpr int FTX = 16, FTY = 16;
+constexpr int BADJ_NUM_SHAPES = 2;
+constexpr int BADJ_SHAPES[BADJ_NUM_SHAPES][2] = {{16, 16}, {32, 8}};
+
+using BFwdFn = void (*)(const BAdjMember *, const BTile *, int, int, int, int, int, int);
+using BRevAFn = void (*)(const BAdjMember *, const BTile *, int, int, int);
+using BRevBFn = void (*)(const BAdjMember *, const BTile *, int, int, int);
+
+template <int S>
+constexpr BFwdFn fwd_of(int limiter) {
+    return limiter == LIM_MC ? badj_forward_kernel<S, LIM_MC, FTX, FTY>
+                             : badj_forward_kernel<S, LIM_VAN_LEER, FTX, FTY>;
+}
+BFwdFn fwd_fn(int solver, int limiter) {
+    return solver == RUSANOV ? fwd_of<RUSANOV>(limiter)
+         : solver == HLLE    ? fwd_of<HLLE>(limiter)
+                             : fwd_of<HLLD>(limiter);
+}
+
+template <int S, int TX, int TY>
+constexpr BRevAFn reva_of(int limiter) {
+    return limiter == LIM_MC ? badj_rev_a_kernel<S, LIM_MC, TX, TY>
+                             : badj_rev_a_kernel<S, LIM_VAN_LEER, TX, TY>;
+}
+BRevAFn reva_fn(int solver, int limiter, int shape) {
+    if (shape == 0)
+        return solver == RUSANOV ? reva_of<RUSANOV, 16, 16>(limiter)
+             : solver == HLLE    ? reva_of<HLLE, 16, 16>(limiter)
+                                 : reva_of<HLLD, 16, 16>(limiter);
+    return solver == RUSANOV ? reva_of<RUSANOV, 32, 8>(limiter)
+         : solver == HLLE    ? reva_of<HLLE, 32, 8>(limiter)
+                             : reva_of<HLLD, 32, 8>(limiter);
+}
+BRevBFn revb_fn(int limiter, int shape) {
+    if (shape == 0)
+        return limiter == LIM_MC ? badj_rev_b_kernel<LIM_MC, 16, 16>
+                                 : badj_rev_b_kernel<LIM_VAN_LEER, 16, 16>;
+    return limiter == LIM_MC ? badj_rev_b_kernel<LIM_MC, 32, 8>
+                             : badj_rev_b_kernel<LIM_VAN_LEER, 32, 8>;
+}
+
+}  // namespace
+}  // namespace chimera
+
+using namespace chimera;
+
+struct chimera_badj {
+    int members = 0;
+    chimera_scheme_t sch{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<BAdjMember> hmem;
+    BAdjMember *dmem = nullptr;
+    double *arena = nullptr;       // per member: lam buffers and exchange arrays
+    std::vector<double *> traj;    // per member trajectory (grown on demand)
+    int traj_cap = 0;              // steps the trajectories hold
+    double *dts = nullptr;         // members x dt_cap
+    int dt_cap = 0;
+    BTile *dtiles[3] = {};         // forward, reverse shape 0, reverse shape 1
+    int ntiles[3] = {};
+    int shape_a = 1, shape_b = 1;
+    int cu_count = 0;
+    long long cells = 0;
+};
+
+namespace {
+
+int badj_grid(chimera_badj *b, const void *fn, int threads, int &grid) {
+    int occ = 0;
+    CH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, 0));
+    grid = std::max(8, b->cu_count * std::max(occ, 1) / 8 * 8);
+    return CHIMERA_OK;
+}
+
+int badj_tables(chimera_badj *b) {
+    std::vector<int> order(b->members);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return b->hmem[x].n > b->hmem[y].n;
+    });
+    const int shapes[3][2] = {{FTX, FTY}, {BADJ_SHAPES[0][0], BADJ_SHAPES[0][1]},
+                              {BADJ_SHAPES[1][0], BADJ_SHAPES[1][1]}};
+    for (int sh = 0; sh < 3; ++sh) {
+        std::vector<BTile> t;
+        for (int m : order) {
+            const int tx = (b->hmem[m].G.nx + shapes[sh][0] - 1) / shapes[sh][0];
+            const int ty = (b->hmem[m].G.ny + shapes[sh][1] - 1) / shapes[sh][1];
+            for (int y = 0; y < ty; ++y)
+                for (int x = 0; x < tx; ++x) t.push_back({m, x, y});
+        }
+        b->ntiles[sh] = (int)t.size();
+        CH_HIP(hipMalloc(&b->dtiles[sh], t.size() * sizeof(BTile)));
+        CH_HIP(hipMemcpy(b->dtiles[sh], t.data(), t.size() * sizeof(BTile), hipMemcpyHostToDevice));
+    }
+    return CHIMERA_OK;
+}
+
+// per-call pointers (state0, lam, state_out, dts) into the device member table
+int badj_upload(chimera_badj *b, const double *const *state0, double *const *lam,
+                double *const *state_out) {
+    for (int m = 0; m < b->members; ++m) {
+        BAdjMember &h = b->hmem[m];
+        for (int q = 0; q < 6; ++q) {
+            h.state0[q] = state0 ? state0[6 * m + q] : nullptr;
+            h.lam_io[q] = lam ? lam[6 * m + q] : nullptr;
+            h.state_out[q] = state_out ? state_out[6 * m + q] : nullptr;
+        }
+        h.traj = b->traj[m];
+        h.dts = b->dts + (size_t)m * b->dt_cap;
+    }
+    CH_HIP(hipStreamSynchronize(b->stream));
+    CH_HIP(hipMemcpy(b->dmem, b->hmem.data(), b->hmem.size() * sizeof(BAdjMember),
+                     hipMemcpyHostToDevice));
+    return CHIMERA_OK;
+}
+
+// dts (host, members x nsteps) to the device, and trajectories for nsteps steps
+int badj_prepare(chimera_badj *b, const double *dt_in, int nsteps) {
+    CH_HIP(hipStreamSynchronize(b->stream));
+    if (nsteps > b->dt_cap) {
+        (void)hipFree(b->dts);
+        b->dts = nullptr;
+        b->dt_cap = 0;
+        CH_HIP(hipMalloc(&b->dts, (size_t)b->members * nsteps * sizeof(double)));
+        b->dt_cap = nsteps;
+    }
+    std::vector<double> h((size_t)b->members * b->dt_cap, 0.0);
+    for (int m = 0; m < b->members; ++m)
+        for (int s = 0; s < nsteps; ++s) h[(size_t)m * b->dt_cap + s] = dt_in[(size_t)m * nsteps + s];
+    CH_HIP(hipMemcpy(b->dts, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (nsteps > b->traj_cap) {
+        for (auto &p : b->traj) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        b->traj_cap = 0;
+        for (int m = 0; m < b->members; ++m) {
+            const size_t bytes = (size_t)3 * nsteps * 6 * b->hmem[m].n * sizeof(double);
+            if (hipMalloc(&b->traj[m], bytes) != hipSuccess) {
+                set_error("batched adjoint: trajectory of member %d (%zu bytes) failed", m, bytes);
+                return CHIMERA_ENOMEM;
+            }
+        }
+        b->traj_cap = nsteps;
+    }
+    return CHIMERA_OK;
+}
+
+// nsteps forward steps of every member from state0; stage inputs kept in the
+// trajectories; the last step's output to state_out when given (else not computed)
+int badj_forward(chimera_badj *b, int nsteps, bool to_out) {
+    const BFwdFn fn = fwd_fn(b->sch.riemann_solver, b->sch.slope_limiter);
+    int grid;
+    CH_TRY(badj_grid(b, (const void *)fn, STAGE_THREADS, grid));
+    for (int s = 0; s < nsteps; ++s) {
+        const bool last = s + 1 == nsteps;
+        for (int st = 1; st <= 3; ++st) {
+            if (st == 3 && last && !to_out) break;
+            hipLaunchKernelGGL(fn, dim3(grid), dim3(STAGE_THREADS), 0, b->stream, b->dmem,
+                               b->dtiles[0], b->ntiles[0], s, st, (st == 3 && last) ? 1 : 0,
+                               b->sch.use_slope_limiting, b->sch.fallback_2_mood);
+            CH_TRY(launch_check("badj_forward_kernel"));
+        }
+    }
+    return CHIMERA_OK;
+}
+
+}  // namespace
+
+extern "C" int chimera_badj_destroy(chimera_badj_t *b) {
+    if (!b) return CHIMERA_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto p : b->traj) (void)hipFree(p);
+    for (auto p : b->dtiles) (void)hipFree(p);
+    (void)hipFree(b->arena);
+    (void)hipFree(b->dmem);
+    (void)hipFree(b->dts);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return CHIMERA_OK;
+}
+
+extern "C" int chimera_badj_create(chimera_badj_t **out, int members, const int *nx, const int *ny,
+                                   const double *dx, const double *gamma, const chimera_bc_t *bc,
+                                   const chimera_scheme_t *scheme, void *stream) {
+    CH_REQUIRE(out && nx && ny && dx && gamma && bc && scheme, "NULL argument");
+    CH_REQUIRE(members >= 1, "members must be >= 1");
+    CH_TRY(check_scheme(scheme));
+    chimera_badj *b = new chimera_badj();
+    b->members = members;
+    b->sch = *scheme;
+    b->hmem.resize(members);
+    b->traj.assign(members, nullptr);
+    size_t total = 0;
+    for (int m = 0; m < members; ++m) {
+        if (!(nx[m] >= 4 && ny[m] >= 4 && dx[m] > 0.0 && gamma[m] > 1.0)) {
+            set_error("member %d: need nx, ny >= 4, dx > 0 and gamma > 1", m);
+            delete b;
+            return CHIMERA_EINVAL;
+        }
+        int px = 0, py = 0;
+        const int rc = check_bc(&bc[m], &px, &py);
+        if (rc != CHIMERA_OK) {
+            delete b;
+            return rc;
+        }
+        BAdjMember &h = b->hmem[m];
+        h = BAdjMember{};
+        h.G = AdjGrid{nx[m], ny[m], px, py, dx[m], dx[m] * dx[m], gamma[m],
+                      scheme->use_slope_limiting, scheme->fallback_2_mood};
+        h.n = (size_t)nx[m] * ny[m];
+        total += h.n * (18 + FusedBufs::ARRAYS);
+        b->cells += (long long)h.n;
+    }
+    if (stream) {
+        b->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete b;
+            set_error("hipStreamCreate failed");
+            return CHIMERA_EHIP;
+        }
+        b->own_stream = true;
+    }
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        hipMalloc(&b->arena, total * sizeof(double)) != hipSuccess ||
+        hipMalloc(&b->dmem, members * sizeof(BAdjMember)) != hipSuccess) {
+        set_error("batched adjoint: device set-up (%zu doubles of workspace) failed", total);
+        chimera_badj_destroy(b);
+        return CHIMERA_ENOMEM;
+    }
+    b->cu_count = prop.multiProcessorCount;
+    double *p = b->arena;
+    for (auto &h : b->hmem) {
+        for (auto &row : h.lam)
+            for (auto &x : row) {
+                x = p;
+                p += h.n;
+            }
+        h.F = FusedBufs{p, h.n};
+        p += (size_t)FusedBufs::ARRAYS * h.n;
+    }
+    int rc = badj_tables(b);
+    if (rc != CHIMERA_OK) {
+        chimera_badj_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return CHIMERA_OK;
+}
+
+// Every member nsteps steps forward with its prescribed dts (dt_in: members x nsteps,
+// host), from state_in to state_out (members x 6 device pointers; may be equal).
+extern "C" int chimera_badj_forward(chimera_badj_t *b, const double *const *state_in,
+                                    const double *dt_in, int nsteps, double *const *state_out) {
+    CH_REQUIRE(b && state_in && dt_in && state_out, "NULL argument");
+    CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
+    for (long long k = 0; k < (long long)b->members * nsteps; ++k)
+        CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
+    CH_TRY(badj_prepare(b, dt_in, nsteps));
+    CH_TRY(badj_upload(b, state_in, nullptr, state_out));
+    CH_TRY(badj_forward(b, nsteps, true));
+    CH_HIP(hipStreamSynchronize(b->stream));
+    return CHIMERA_OK;
+}
+
+// Reverse sweep over one checkpoint segment for every member: recompute the segment
+// from state0 (members x 6 device pointers) with the dts dt_in (members x nsteps,
+// host), then lam <- (dU^{nsteps}/dU^0)^T lam in place (members x 6 device pointers).
+extern "C" int chimera_badj_segment_vjp(chimera_badj_t *b, const double *const *state0,
+                                        const double *dt_in, int nsteps, double *const *lam) {
+    CH_REQUIRE(b && state0 && dt_in && lam, "NULL argument");
+    CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
+    for (long long k = 0; k < (long long)b->members * nsteps; ++k)
+        CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
+    CH_TRY(badj_prepare(b, dt_in, nsteps));
+    CH_TRY(badj_upload(b, state0, lam, nullptr));
+    CH_TRY(badj_forward(b, nsteps, false));
+    const BRevAFn fa = reva_fn(b->sch.riemann_solver, b->sch.slope_limiter, b->shape_a);
+    const BRevBFn fb = revb_fn(b->sch.slope_limiter, b->shape_b);
+    const int ta = BADJ_SHAPES[b->shape_a][0] * BADJ_SHAPES[b->shape_a][1];
+    const int tb = BADJ_SHAPES[b->shape_b][0] * BADJ_SHAPES[b->shape_b][1];
+    int ga, gb;
+    CH_TRY(badj_grid(b, (const void *)fa, ta, ga));
+    CH_TRY(badj_grid(b, (const void *)fb, tb, gb));
+    for (int s = nsteps - 1; s >= 0; --s)
+        for (int st = 3; st >= 1; --st) {
+            hipLaunchKernelGGL(fa, dim3(ga), dim3(ta), 0, b->stream, b->dmem,
+                               b->dtiles[1 + b->shape_a], b->ntiles[1 + b->shape_a], s, st);
+            hipLaunchKernelGGL(fb, dim3(gb), dim3(tb), 0, b->stream, b->dmem,
+                               b->dtiles[1 + b->shape_b], b->ntiles[1 + b->shape_b], s, st);
+            CH_TRY(launch_check("batched adjoint reverse stage"));
+        }
+    CH_HIP(hipStreamSynchronize(b->stream));
+    return CHIMERA_OK;
+}
+
+// Tile shapes of the two reverse passes: 0 = 16 x 16, 1 = 32 x 8 (-1 keeps).
+extern "C" int chimera_badj_set_tiles(chimera_badj_t *b, int shape_a, int shape_b) {
+    CH_REQUIRE(b, "NULL batched adjoint");
+    CH_REQUIRE(shape_a >= -1 && shape_a < BADJ_NUM_SHAPES && shape_b >= -1 &&
+                   shape_b < BADJ_NUM_SHAPES,
+               "tile shapes must be -1, 0 or 1, got %d, %d", shape_a, shape_b);
+    if (shape_a >= 0) b->shape_a = shape_a;
+    if (shape_b >= 0) b->shape_b = shape_b;
+    return CHIMERA_OK;
+}
+
+extern "C" void *chimera_badj_stream(chimera_badj_t *b) { return b ? (void *)b->stream : nullptr; }

@@ -289,6 +289,31 @@ int chimera_adj_set_tiles(chimera_adj_t *adj, int shape_a, int shape_b);
 int chimera_face_vjp(int solver, int method, int n, double gamma, const double *in,
                      const double *lam_flux, double *out);
 
+/* ---- batched adjoint (ensembles of uneven grids) -------------------------------------
+ * The adjoint of chimera_adj_* for many independent grids at once: every forward and
+ * reverse stage is one persistent launch over a tile table spanning all members.  Each
+ * member has its own size, dx, gamma, BCs and prescribed dt sequence; the scheme is
+ * shared.  Per-member results are bit-identical to the single-grid adjoint.  State and
+ * lam arguments are members x 6 device pointers (member m's field q at [6*m + q]); dt
+ * arrays are members x nsteps host doubles (member m's steps at [m*nsteps ..]).  No
+ * counterpart in the reference (it has no derivatives). */
+typedef struct chimera_badj chimera_badj_t;
+int chimera_badj_create(chimera_badj_t **badj, int members, const int *nx, const int *ny,
+                        const double *dx, const double *gamma, const chimera_bc_t *bc,
+                        const chimera_scheme_t *scheme, void *stream);
+int chimera_badj_destroy(chimera_badj_t *badj);
+/* nsteps steps of every member from state_in to state_out (may be the same arrays). */
+int chimera_badj_forward(chimera_badj_t *badj, const double *const *state_in, const double *dt_in,
+                         int nsteps, double *const *state_out);
+/* One checkpoint segment of every member: recompute from state0 keeping every stage
+ * input (3 * nsteps states per member, allocated on first use and kept), then
+ * lam <- (dU^{nsteps}/dU^0)^T lam in place. */
+int chimera_badj_segment_vjp(chimera_badj_t *badj, const double *const *state0,
+                             const double *dt_in, int nsteps, double *const *lam);
+/* Tile shapes of the two reverse passes: 0 = 16x16, 1 = 32x8 (default; -1 keeps). */
+int chimera_badj_set_tiles(chimera_badj_t *badj, int shape_a, int shape_b);
+void *chimera_badj_stream(chimera_badj_t *badj);
+
 #ifdef __cplusplus
 }
 #endif
