@@ -76,12 +76,15 @@ __device__ __forceinline__ GridView member_view(const BAdjMember &m, int s, int 
     return g;
 }
 
-// Forward stage st of step s for every member; the output goes to the trajectory (or,
-// with to_out, to the member's state_out).
+// Forward stage st of step s for every member.  Segment recompute (pure = 0): the
+// stage output goes to the trajectory, which keeps every stage input.  Plain forward
+// (pure = 1): the running state is the member's state_out (U^0 read from state0), the
+// two intermediate stage states use trajectory slots 1 and 2, and stage 3 overwrites
+// state_out in place (it reads u^n only at the cell it writes).
 template <int SOLVER, int LIMITER, int TX, int TY>
 __global__ __launch_bounds__(STAGE_THREADS) void badj_forward_kernel(
     const BAdjMember *__restrict__ mem, const BTile *__restrict__ tiles, int ntiles, int s, int st,
-    int to_out, int use_lim, int mood) {
+    int pure, int use_lim, int mood) {
     __shared__ double lds[TileShape<TX, TY>::LDS_DOUBLES];
     int begin, end, slot, slots;
     badj_segment(ntiles, begin, end, slot, slots);
@@ -90,8 +93,16 @@ __global__ __launch_bounds__(STAGE_THREADS) void badj_forward_kernel(
         const BAdjMember &m = mem[tr.member];
         GridView g = member_view(m, s, st);
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
-            g.uout[q] = to_out ? m.state_out[q] : slot_field(m, 3 * s + st, q);
+        for (int q = 0; q < 6; ++q) {
+            if (pure) {
+                const double *un = s == 0 ? m.state0[q] : m.state_out[q];
+                g.uin[q] = st == 1 ? un : slot_field(m, st - 1, q);
+                g.u0[q] = un;
+                g.uout[q] = st == 3 ? m.state_out[q] : slot_field(m, st, q);
+            } else {
+                g.uout[q] = slot_field(m, 3 * s + st, q);
+            }
+        }
         stage_tile<SOLVER, LIMITER, TX, TY>(g, st, use_lim != 0, mood != 0, tr.tx, tr.ty, lds);
         __syncthreads();   // the next tile reuses LDS
     }
@@ -269,8 +280,8 @@ int badj_upload(chimera_badj *b, const double *const *state0, double *const *lam
     return CHIMERA_OK;
 }
 
-// dts (host, members x nsteps) to the device, and trajectories for nsteps steps
-int badj_prepare(chimera_badj *b, const double *dt_in, int nsteps) {
+// dts (host, members x nsteps) to the device, and trajectories for traj_steps steps
+int badj_prepare(chimera_badj *b, const double *dt_in, int nsteps, int traj_steps) {
     CH_HIP(hipStreamSynchronize(b->stream));
     if (nsteps > b->dt_cap) {
         (void)hipFree(b->dts);
@@ -283,36 +294,37 @@ int badj_prepare(chimera_badj *b, const double *dt_in, int nsteps) {
     for (int m = 0; m < b->members; ++m)
         for (int s = 0; s < nsteps; ++s) h[(size_t)m * b->dt_cap + s] = dt_in[(size_t)m * nsteps + s];
     CH_HIP(hipMemcpy(b->dts, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-    if (nsteps > b->traj_cap) {
+    if (traj_steps > b->traj_cap) {
         for (auto &p : b->traj) {
             (void)hipFree(p);
             p = nullptr;
         }
         b->traj_cap = 0;
         for (int m = 0; m < b->members; ++m) {
-            const size_t bytes = (size_t)3 * nsteps * 6 * b->hmem[m].n * sizeof(double);
+            const size_t bytes = (size_t)3 * traj_steps * 6 * b->hmem[m].n * sizeof(double);
             if (hipMalloc(&b->traj[m], bytes) != hipSuccess) {
                 set_error("batched adjoint: trajectory of member %d (%zu bytes) failed", m, bytes);
                 return CHIMERA_ENOMEM;
             }
         }
-        b->traj_cap = nsteps;
+        b->traj_cap = traj_steps;
     }
     return CHIMERA_OK;
 }
 
-// nsteps forward steps of every member from state0; stage inputs kept in the
-// trajectories; the last step's output to state_out when given (else not computed)
-int badj_forward(chimera_badj *b, int nsteps, bool to_out) {
+// nsteps forward steps of every member from state0.  pure: into state_out, keeping
+// nothing; else the segment recompute, keeping every stage input in the trajectories
+// (the last step's third stage, whose output the reverse does not need, is skipped).
+int badj_forward(chimera_badj *b, int nsteps, bool pure) {
     const BFwdFn fn = fwd_fn(b->sch.riemann_solver, b->sch.slope_limiter);
     int grid;
     CH_TRY(badj_grid(b, (const void *)fn, STAGE_THREADS, grid));
     for (int s = 0; s < nsteps; ++s) {
         const bool last = s + 1 == nsteps;
         for (int st = 1; st <= 3; ++st) {
-            if (st == 3 && last && !to_out) break;
+            if (st == 3 && last && !pure) break;
             hipLaunchKernelGGL(fn, dim3(grid), dim3(STAGE_THREADS), 0, b->stream, b->dmem,
-                               b->dtiles[0], b->ntiles[0], s, st, (st == 3 && last) ? 1 : 0,
+                               b->dtiles[0], b->ntiles[0], s, st, pure ? 1 : 0,
                                b->sch.use_slope_limiting, b->sch.fallback_2_mood);
             CH_TRY(launch_check("badj_forward_kernel"));
         }
@@ -414,7 +426,7 @@ extern "C" int chimera_badj_forward(chimera_badj_t *b, const double *const *stat
     CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
     for (long long k = 0; k < (long long)b->members * nsteps; ++k)
         CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
-    CH_TRY(badj_prepare(b, dt_in, nsteps));
+    CH_TRY(badj_prepare(b, dt_in, nsteps, 1));   // slots 1 and 2 of one step
     CH_TRY(badj_upload(b, state_in, nullptr, state_out));
     CH_TRY(badj_forward(b, nsteps, true));
     CH_HIP(hipStreamSynchronize(b->stream));
@@ -430,7 +442,7 @@ extern "C" int chimera_badj_segment_vjp(chimera_badj_t *b, const double *const *
     CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
     for (long long k = 0; k < (long long)b->members * nsteps; ++k)
         CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
-    CH_TRY(badj_prepare(b, dt_in, nsteps));
+    CH_TRY(badj_prepare(b, dt_in, nsteps, nsteps));
     CH_TRY(badj_upload(b, state0, lam, nullptr));
     CH_TRY(badj_forward(b, nsteps, false));
     const BRevAFn fa = reva_fn(b->sch.riemann_solver, b->sch.slope_limiter, b->shape_a);

@@ -302,7 +302,8 @@ int chimera_badj_create(chimera_badj_t **badj, int members, const int *nx, const
                         const double *dx, const double *gamma, const chimera_bc_t *bc,
                         const chimera_scheme_t *scheme, void *stream);
 int chimera_badj_destroy(chimera_badj_t *badj);
-/* nsteps steps of every member from state_in to state_out (may be the same arrays). */
+/* nsteps steps of every member from state_in to state_out (may be the same arrays);
+ * keeps no trajectory (two intermediate stage states per member). */
 int chimera_badj_forward(chimera_badj_t *badj, const double *const *state_in, const double *dt_in,
                          int nsteps, double *const *state_out);
 /* One checkpoint segment of every member: recompute from state0 keeping every stage
