@@ -75,6 +75,20 @@ class AdjointStepper:
                                               len(dts), _ptrs(lam)), "chimera_adj_segment_vjp")
         return lam
 
+    def sweep(self, ckpts, dts, seg_steps: int, lam: torch.Tensor) -> torch.Tensor:
+        """lam <- (dU_N/dU_0)^T lam in place for a run of len(dts) steps checkpointed
+        every seg_steps steps (ckpts[k] = U at step k * seg_steps): every segment
+        recomputed and reversed, the recompute of segment k-1 overlapping the reverse
+        of segment k."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        nseg = -(-len(dts) // seg_steps)
+        if len(ckpts) != nseg:
+            raise ValueError(f"{len(ckpts)} checkpoints for {nseg} segments")
+        check(self._L.chimera_adj_sweep(self._h, _ptrs_many(ckpts),
+                                        dts.ctypes.data_as(C.POINTER(C.c_double)), len(dts),
+                                        int(seg_steps), _ptrs(lam)), "chimera_adj_sweep")
+        return lam
+
     def close(self):
         if getattr(self, "_h", None):
             self._L.chimera_adj_destroy(self._h)
@@ -122,14 +136,10 @@ class _Evolve(torch.autograd.Function):
         ckpts = ctx.saved_tensors
         dts, K = ctx.dts, ctx.K
         nx, ny, dx, gamma, bc, scheme = ctx.geom
-        n = len(dts)
         adj = AdjointStepper(nx, ny, dx, gamma, bc, scheme)
         lam = gN.contiguous().clone()
         try:
-            for seg in reversed(range(len(ckpts))):
-                s0 = seg * K
-                s1 = min(s0 + K, n)
-                adj.segment_vjp(ckpts[seg], dts[s0:s1], lam)
+            adj.sweep(ckpts, dts, K, lam)
         finally:
             adj.close()
         return lam, None, None, None

@@ -11,8 +11,14 @@
 // forward operation that read it), so there are no atomics and the gradient is
 // deterministic.  Branches (limiter cases, MOOD resets, face and conserved floors,
 // thermal resets, Riemann-solver regions and catches) are the ones the forward pass
-// took.  Rusanov and HLLE are reversed by hand (riemann_adj.hpp); HLLD Jacobians come
-// from forward-mode dual numbers (riemann_dual.hpp).
+// took.  The three Riemann solvers are reversed by hand (riemann_adj.hpp); their
+// forward-mode dual-number VJPs (riemann_dual.hpp) are the cross-check.
+//
+// A whole reverse sweep (chimera_adj_sweep) overlaps the recompute of checkpoint
+// segment k-1, on a second stream, with the reverse of segment k: the reverse kernels
+// leave most of the machine idle at sizes where one thread per cell gives a few waves
+// per SIMD.
+#include <algorithm>
 #include <vector>
 
 #include "adjoint_phys.hpp"
@@ -428,6 +434,12 @@ struct chimera_adj {
     double *traj = nullptr;       // traj_cap x 6 fields
     double **traj_tab = nullptr;  // device tables, 6 pointers per slot
     int traj_cap = 0;
+    // the whole-run sweep: segment k-1 is recomputed on rstream while segment k is
+    // reversed on stream (two trajectory halves, alternating by segment parity)
+    hipStream_t rstream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_traj[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
+    double **ck_tab = nullptr;    // device tables of the checkpoints, 6 pointers each
+    int ck_cap = 0;
 };
 
 namespace {
@@ -506,10 +518,10 @@ GridView grid_view(const chimera_adj *a, const Ptr6 &in, const Ptr6 &u0, const P
 // One stage through the fused kernel; with store, keeping the intermediates in
 // S[stage - 1].
 int run_forward_stage(chimera_adj *a, int stage, const Ptr6 &in, const Ptr6 &u0, const Ptr6 &out,
-                      const double *dt_dev, bool store = true) {
+                      const double *dt_dev, bool store = true, hipStream_t st = nullptr) {
     GridView g = grid_view(a, in, u0, out, dt_dev);
     hipLaunchKernelGGL(adj_stage_fn(a->sch.riemann_solver, a->sch.slope_limiter, store),
-                       dim3(a->grid), dim3(STAGE_THREADS), 0, a->stream, g, a->S[stage - 1], stage,
+                       dim3(a->grid), dim3(STAGE_THREADS), 0, st ? st : a->stream, g, a->S[stage - 1], stage,
                        a->G.use_lim, a->G.mood, a->tiles_x, a->ntiles);
     return launch_check("adjoint forward stage");
 }
@@ -662,6 +674,73 @@ Ptr6 traj_slot(chimera_adj *a, int s) {
     return r;
 }
 
+Tab slot_tab(chimera_adj *a, int s) { return (Tab)(a->traj_tab + 6 * (size_t)s); }
+
+// Recompute one checkpoint segment from h0 on stream st, keeping every stage input in
+// the trajectory slots from base on: slot base+3s holds U^s (s >= 1; U^0 is h0), slots
+// base+3s+1 and base+3s+2 the stage outputs U^(1), U^(2).  dt_dev: the segment's dts.
+int recompute_segment(chimera_adj *a, hipStream_t st, const Ptr6 &h0, const double *dt_dev,
+                      int nsteps, int base) {
+    for (int s = 0; s < nsteps; ++s) {
+        const Ptr6 in = s == 0 ? h0 : traj_slot(a, base + 3 * s);
+        const double *dts = dt_dev + s;
+        CH_TRY(run_forward_stage(a, 1, in, in, traj_slot(a, base + 3 * s + 1), dts, false, st));
+        CH_TRY(run_forward_stage(a, 2, traj_slot(a, base + 3 * s + 1), in,
+                                 traj_slot(a, base + 3 * s + 2), dts, false, st));
+        if (s + 1 < nsteps)
+            CH_TRY(run_forward_stage(a, 3, traj_slot(a, base + 3 * s + 2), in,
+                                     traj_slot(a, base + 3 * s + 3), dts, false, st));
+    }
+    return CHIMERA_OK;
+}
+
+// Reverse of a recomputed segment on a->stream (h0 / t0: its checkpoint as host pointer
+// set and device table).  lam ping-pongs between cur and other; on return cur holds
+// lam at the segment's start.
+int reverse_segment(chimera_adj *a, const Ptr6 &h0, Tab t0, const double *dt_dev,
+                    const double *dt_host, int nsteps, int base, Tab &cur, Tab &other) {
+    for (int s = nsteps - 1; s >= 0; --s) {
+        const Ptr6 hs = s == 0 ? h0 : traj_slot(a, base + 3 * s);
+        Tab ts = s == 0 ? t0 : slot_tab(a, base + 3 * s);
+        const Ptr6 s1 = traj_slot(a, base + 3 * s + 1), s2 = traj_slot(a, base + 3 * s + 2);
+        CH_TRY(vjp_core(a, hs, ts, dt_dev + s, dt_host[s], cur, other, &s1,
+                        slot_tab(a, base + 3 * s + 1), &s2, slot_tab(a, base + 3 * s + 2)));
+        Tab t = cur;
+        cur = other;
+        other = t;
+    }
+    return CHIMERA_OK;
+}
+
+// lam back into the caller's arrays when the ping-pong ended in lam_c
+int finish_lam(chimera_adj *a, Tab cur, double *const *lam) {
+    if (cur != table(a, TB_LIN)) {
+        const size_t bytes = a->n * sizeof(double);
+        for (int q = 0; q < 6; ++q)
+            CH_HIP(hipMemcpyAsync(lam[q], a->lam_c.p[q], bytes, hipMemcpyDeviceToDevice, a->stream));
+    }
+    return CHIMERA_OK;
+}
+
+int ensure_sweep(chimera_adj *a, int nseg) {
+    if (!a->rstream) {
+        CH_HIP(hipStreamCreateWithFlags(&a->rstream, hipStreamNonBlocking));
+        CH_HIP(hipEventCreateWithFlags(&a->ev_start, hipEventDisableTiming));
+        for (int b = 0; b < 2; ++b) {
+            CH_HIP(hipEventCreateWithFlags(&a->ev_traj[b], hipEventDisableTiming));
+            CH_HIP(hipEventCreateWithFlags(&a->ev_rev[b], hipEventDisableTiming));
+        }
+    }
+    if (nseg > a->ck_cap) {
+        (void)hipFree(a->ck_tab);
+        a->ck_tab = nullptr;
+        a->ck_cap = 0;
+        CH_HIP(hipMalloc(&a->ck_tab, (size_t)nseg * 6 * sizeof(double *)));
+        a->ck_cap = nseg;
+    }
+    return CHIMERA_OK;
+}
+
 }  // namespace
 
 extern "C" int chimera_adj_destroy(chimera_adj_t *a) {
@@ -673,6 +752,13 @@ extern "C" int chimera_adj_destroy(chimera_adj_t *a) {
     (void)hipFree(a->dt_dev);
     (void)hipFree(a->traj);
     (void)hipFree(a->traj_tab);
+    (void)hipFree(a->ck_tab);
+    if (a->rstream) {
+        (void)hipStreamSynchronize(a->rstream);
+        (void)hipStreamDestroy(a->rstream);
+    }
+    for (hipEvent_t e : {a->ev_start, a->ev_traj[0], a->ev_traj[1], a->ev_rev[0], a->ev_rev[1]})
+        if (e) (void)hipEventDestroy(e);
     if (a->own_stream && a->stream) (void)hipStreamDestroy(a->stream);
     delete a;
     return CHIMERA_OK;
@@ -820,38 +906,59 @@ extern "C" int chimera_adj_segment_vjp(chimera_adj_t *a, const double *const *st
     CH_TRY(set_dts(a, dt_in, nsteps));
     CH_TRY(set_table(a, TB_IN, state0));
     CH_TRY(set_table(a, TB_LIN, (const double *const *)lam));
-    // recompute the segment, keeping every stage input: slot 3s holds U^s (s >= 1;
-    // U^0 is the caller's state0), slots 3s+1 and 3s+2 the stage outputs U^(1), U^(2)
     const Ptr6 h0 = ptr6(state0);
-    for (int s = 0; s < nsteps; ++s) {
-        const Ptr6 in = s == 0 ? h0 : traj_slot(a, 3 * s);
-        const double *dts = a->dt_dev + s;
-        CH_TRY(run_forward_stage(a, 1, in, in, traj_slot(a, 3 * s + 1), dts, false));
-        CH_TRY(run_forward_stage(a, 2, traj_slot(a, 3 * s + 1), in, traj_slot(a, 3 * s + 2), dts,
-                                 false));
-        if (s + 1 < nsteps)
-            CH_TRY(run_forward_stage(a, 3, traj_slot(a, 3 * s + 2), in, traj_slot(a, 3 * s + 3), dts,
-                                     false));
-    }
-    auto slot_tab = [&](int k) { return (Tab)(a->traj_tab + 6 * (size_t)k); };
+    CH_TRY(recompute_segment(a, a->stream, h0, a->dt_dev, nsteps, 0));
     // lam ping-pongs between the caller's arrays and lam_c
     Tab cur = table(a, TB_LIN), other = table(a, TB_LC);
-    for (int s = nsteps - 1; s >= 0; --s) {
-        const Ptr6 hs = s == 0 ? h0 : traj_slot(a, 3 * s);
-        Tab ts = s == 0 ? table(a, TB_IN) : slot_tab(3 * s);
-        const Ptr6 s1 = traj_slot(a, 3 * s + 1), s2 = traj_slot(a, 3 * s + 2);
-        CH_TRY(vjp_core(a, hs, ts, a->dt_dev + s, dt_in[s], cur, other, &s1, slot_tab(3 * s + 1),
-                        &s2, slot_tab(3 * s + 2)));
-        Tab t = cur;
-        cur = other;
-        other = t;
-    }
-    if (cur != table(a, TB_LIN)) {
-        size_t bytes = a->n * sizeof(double);
-        for (int q = 0; q < 6; ++q)
-            CH_HIP(hipMemcpyAsync(lam[q], a->lam_c.p[q], bytes, hipMemcpyDeviceToDevice, a->stream));
-    }
+    CH_TRY(reverse_segment(a, h0, table(a, TB_IN), a->dt_dev, dt_in, nsteps, 0, cur, other));
+    CH_TRY(finish_lam(a, cur, lam));
     CH_HIP(hipStreamSynchronize(a->stream));
+    return CHIMERA_OK;
+}
+
+extern "C" int chimera_adj_sweep(chimera_adj_t *a, const double *const *states, const double *dt_in,
+                                 int nsteps, int seg_steps, double *const *lam) {
+    CH_REQUIRE(a && states && dt_in && lam, "NULL argument");
+    CH_REQUIRE(nsteps >= 1 && seg_steps >= 1, "nsteps and seg_steps must be >= 1");
+    for (int s = 0; s < nsteps; ++s)
+        CH_REQUIRE(dt_in[s] > 0.0, "dt_in[%d] = %g is not positive", s, dt_in[s]);
+    const int nseg = (nsteps + seg_steps - 1) / seg_steps;
+    for (int k = 0; k < 6 * nseg; ++k) CH_REQUIRE(states[k], "states[%d] is NULL", k);
+    const int K = std::min(seg_steps, nsteps);
+    // one segment: the trajectory halves alternate, so two are enough
+    CH_TRY(ensure_traj(a, (nseg > 1 ? 2 : 1) * 3 * K));
+    CH_TRY(ensure_sweep(a, nseg));
+    CH_TRY(set_dts(a, dt_in, nsteps));
+    CH_TRY(set_table(a, TB_LIN, (const double *const *)lam));
+    CH_HIP(hipMemcpy(a->ck_tab, states, (size_t)nseg * 6 * sizeof(double *), hipMemcpyHostToDevice));
+    // the recompute stream starts after everything queued on the main stream
+    CH_HIP(hipEventRecord(a->ev_start, a->stream));
+    CH_HIP(hipStreamWaitEvent(a->rstream, a->ev_start, 0));
+    auto seg_len = [&](int k) { return std::min(seg_steps, nsteps - k * seg_steps); };
+    auto recompute = [&](int k) -> int {
+        const int b = k & 1;
+        CH_TRY(recompute_segment(a, a->rstream, ptr6(states + 6 * (size_t)k),
+                                 a->dt_dev + (size_t)k * seg_steps, seg_len(k), b * 3 * K));
+        CH_HIP(hipEventRecord(a->ev_traj[b], a->rstream));
+        return CHIMERA_OK;
+    };
+    Tab cur = table(a, TB_LIN), other = table(a, TB_LC);
+    CH_TRY(recompute(nseg - 1));
+    for (int k = nseg - 1; k >= 0; --k) {
+        if (k >= 1) {
+            // segment k-1 goes into the half segment k+1 used: after that reverse
+            if (k + 1 <= nseg - 1) CH_HIP(hipStreamWaitEvent(a->rstream, a->ev_rev[(k + 1) & 1], 0));
+            CH_TRY(recompute(k - 1));
+        }
+        CH_HIP(hipStreamWaitEvent(a->stream, a->ev_traj[k & 1], 0));
+        CH_TRY(reverse_segment(a, ptr6(states + 6 * (size_t)k), (Tab)(a->ck_tab + 6 * (size_t)k),
+                               a->dt_dev + (size_t)k * seg_steps, dt_in + (size_t)k * seg_steps,
+                               seg_len(k), (k & 1) * 3 * K, cur, other));
+        CH_HIP(hipEventRecord(a->ev_rev[k & 1], a->stream));
+    }
+    CH_TRY(finish_lam(a, cur, lam));
+    CH_HIP(hipStreamSynchronize(a->stream));
+    CH_HIP(hipStreamSynchronize(a->rstream));
     return CHIMERA_OK;
 }
 

@@ -71,20 +71,22 @@ __device__ __forceinline__ void face_state_reverse(double lam[6], int s, bool to
 //
 // rev_tile_a (one TX x TY tile of owned cells per workgroup) replaces the STORE
 // recompute, ar_cell_tail, ar_corner and ar_faces: it loads the primitives of cells
-// [-2, TX+2] x [-2, TY+2], computes the fluxes of the faces of cells [0, TX] x [0, TY]
-// (forward phase 2), the flux update, floors, outflow correction and blend of those
-// cells with the reverse of the tail (lam of Up into LDS), lam of -Ez at the corners
-// around the owned faces, and finally, per owned cell, the face-state recompute, the
+// [-2, TX+2] x [-2, TY+2] together with lam of -Ez at the corners around the owned
+// faces (one load latency for both), computes the fluxes of the faces of cells
+// [0, TX] x [0, TY] (forward phase 2), the flux update, floors, outflow correction and
+// blend of those cells with the reverse of the tail (lam of Up into LDS), and finally,
+// per owned cell, the face-state recompute, the
 // Riemann VJP and the reverse of the face construction of its x and y faces.  It
 // writes what the slope / primitive reverse needs (stage primitives, lam of the
 // face-state candidates, lam of the cell resets, of the outflow primitives and of the
 // cell B) and the update's and CT's part of lam_in; lam_u0 is accumulated.
 //
-// rev_tile_b replaces ar_slopes, ar_prims and ar_bavg: field by field it gathers the
-// slope cotangents of cells [-1, TX+1] x [-1, TY+1] into LDS, sums each cell's own and
-// its neighbours' parts for cells [0, TX] x [0, TY], then reverses cons_to_prim and the
-// face -> cell B average.  Every value is computed with the arithmetic, order and
-// branch of the unfused kernels, so both paths give the same bits.
+// rev_tile_b replaces ar_slopes, ar_prims and ar_bavg: a few fields per round it
+// gathers the slope cotangents of cells [-1, TX+1] x [-1, TY+1] into LDS, sums each
+// cell's own and its neighbours' parts for cells [0, TX] x [0, TY], then reverses
+// cons_to_prim and the face -> cell B average and writes lam_in once per cell.  Every
+// value is computed with the arithmetic, order and branch of the unfused kernels, so
+// both paths give the same bits.
 
 template <int TX, int TY>
 struct RevShapeA {
@@ -171,19 +173,57 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
         }
     }
 
-    // ---- 1: primitives of cells [-2, TX+2] x [-2, TY+2] ------------------------------
-    for (int idx = tid; idx < A::P_SZ; idx += NT) {
+    // ---- 1: primitives of cells [-2, TX+2] x [-2, TY+2] and lam of -Ez at the corners
+    // [-1, TX) x [-1, TY) (as ar_corner); every load of the phase is issued first ---------
+    constexpr int PPA = (A::P_SZ + NT - 1) / NT, EPA = (A::E_SZ + NT - 1) / NT;
+    double pl[PPA][8];
+    size_t pk[PPA];
+#pragma unroll
+    for (int u = 0; u < PPA; ++u) {
+        const int idx = tid + u * NT;
+        if (idx >= A::P_SZ) break;
         const int lx = idx % A::PW, ly = idx / A::PW;
         const int gi = map_index(x0 - 2 + lx, nx, px);
         const int gj = map_index(y0 - 2 + ly, ny, py);
         const int gim = gi > 0 ? gi - 1 : (px ? nx - 1 : 0);
         const int gjm = gj > 0 ? gj - 1 : (py ? ny - 1 : 0);
         const size_t k = (size_t)gi + (size_t)nx * gj;
-        const double Bx = 0.5 * (g.uin[4][k] + g.uin[4][(size_t)gim + (size_t)nx * gj]);
-        const double By = 0.5 * (g.uin[5][k] + g.uin[5][(size_t)gi + (size_t)nx * gjm]);
+        pk[u] = k;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) pl[u][q] = g.uin[q][k];
+        pl[u][6] = g.uin[4][(size_t)gim + (size_t)nx * gj];
+        pl[u][7] = g.uin[5][(size_t)gi + (size_t)nx * gjm];
+    }
+    // per corner: lam_out of the x faces above and below it, of the y faces right and
+    // left of it, and of the faces a clamped edge repeats
+    double el[EPA][6];
+    {
+        const double *lox = lout[4], *loy = lout[5];
+#pragma unroll
+        for (int u = 0; u < EPA; ++u) {
+            const int e = tid + u * NT;
+            if (e >= A::E_SZ) break;
+            const int gx = x0 + e % A::EW - 1, gy = y0 + e / A::EW - 1;
+            if ((!px && (gx < 0 || gx >= nx)) || (!py && (gy < 0 || gy >= ny))) continue;
+            const int a = map_index(gx, nx, px), bb = map_index(gy, ny, py);
+            const size_t k = (size_t)a + (size_t)nx * bb;
+            el[u][0] = lox[k];
+            el[u][1] = lox[bb + 1 < ny ? k + nx : (size_t)a];
+            el[u][2] = lox[(size_t)a];
+            el[u][3] = loy[k];
+            el[u][4] = loy[a + 1 < nx ? k + 1 : (size_t)nx * bb];
+            el[u][5] = loy[(size_t)nx * bb];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PPA; ++u) {
+        const int idx = tid + u * NT;
+        if (idx >= A::P_SZ) break;
+        const int lx = idx % A::PW, ly = idx / A::PW;
+        const double Bx = 0.5 * (pl[u][4] + pl[u][6]);
+        const double By = 0.5 * (pl[u][5] + pl[u][7]);
         double rho, vx, vy, P;
-        cons_to_prim(g.uin[0][k], g.uin[1][k], g.uin[2][k], g.uin[3][k], Bx, By, gam, vol, rho, vx,
-                     vy, P);
+        cons_to_prim(pl[u][0], pl[u][1], pl[u][2], pl[u][3], Bx, By, gam, vol, rho, vx, vy, P);
         sP[0 * A::P_SZ + idx] = rho;
         sP[1 * A::P_SZ + idx] = vx;
         sP[2 * A::P_SZ + idx] = vy;
@@ -192,13 +232,42 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
         sP[5 * A::P_SZ + idx] = By;
         const int ci = lx - 2, cj = ly - 2;
         if (ci >= 0 && ci < TX && cj >= 0 && cj < TY && x0 + ci < nx && y0 + cj < ny) {
-            const size_t ko = (size_t)(x0 + ci) + (size_t)nx * (y0 + cj);
+            const size_t ko = pk[u];
             F.P(0)[ko] = rho;
             F.P(1)[ko] = vx;
             F.P(2)[ko] = vy;
             F.P(3)[ko] = P;
             F.P(4)[ko] = Bx;
             F.P(5)[ko] = By;
+        }
+    }
+    {
+        const double s = dt / dx;
+        const double cb = stage > 1 ? c1 : 1.0;   // lam of the CT-updated face B
+#pragma unroll
+        for (int u = 0; u < EPA; ++u) {
+            const int e = tid + u * NT;
+            if (e >= A::E_SZ) break;
+            const int gx = x0 + e % A::EW - 1, gy = y0 + e / A::EW - 1;
+            if ((!px && (gx < 0 || gx >= nx)) || (!py && (gy < 0 || gy >= ny))) {
+                sLE[e] = 0.0;
+                continue;
+            }
+            const int a = map_index(gx, nx, px), bb = map_index(gy, ny, py);
+            double acc = 0.0;
+            if (bb >= 1 || py) acc += s * (cb * el[u][0]);
+            if (bb + 1 < ny || py) acc -= s * (cb * el[u][1]);
+            if (!py) {
+                if (bb == 1) acc += s * (cb * el[u][2]);
+                if (bb == 0) acc -= s * (cb * el[u][2]);
+            }
+            if (a >= 1 || px) acc -= s * (cb * el[u][3]);
+            if (a + 1 < nx || px) acc += s * (cb * el[u][4]);
+            if (!px) {
+                if (a == 1) acc -= s * (cb * el[u][5]);
+                if (a == 0) acc += s * (cb * el[u][5]);
+            }
+            sLE[e] = acc;
         }
     }
     __syncthreads();
@@ -339,49 +408,13 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
             lin[5][k] = lb[1];
             F.Bc(0)[k] = lBx;
             F.Bc(1)[k] = lBy;
+            if (xlo_open || ylo_open) {   // zero elsewhere: neither written nor read
 #pragma unroll
-            for (int f = 0; f < 6; ++f) F.Pd(f)[k] = lPd[f];
+                for (int f = 0; f < 6; ++f) F.Pd(f)[k] = lPd[f];
+            }
         }
     }
 
-    // ---- 4: lam of -Ez at corners [-1, TX) x [-1, TY) (as ar_corner) -----------------
-    {
-        const double s = dt / dx;
-        const double cb = stage > 1 ? c1 : 1.0;   // lam of the CT-updated face B
-        const double *lox = lout[4], *loy = lout[5];
-        for (int e = tid; e < A::E_SZ; e += NT) {
-            const int la = e % A::EW - 1, lb_ = e / A::EW - 1;
-            const int gx = x0 + la, gy = y0 + lb_;
-            if ((!px && (gx < 0 || gx >= nx)) || (!py && (gy < 0 || gy >= ny))) {
-                sLE[e] = 0.0;
-                continue;
-            }
-            const int a = map_index(gx, nx, px), bb = map_index(gy, ny, py);
-            const size_t k = (size_t)a + (size_t)nx * bb;
-            double acc = 0.0;
-            if (bb >= 1 || py) acc += s * (cb * lox[k]);
-            {
-                const int jn = bb + 1;
-                if (jn < ny) acc -= s * (cb * lox[k + nx]);
-                else if (py) acc -= s * (cb * lox[(size_t)a]);
-            }
-            if (!py) {
-                if (bb == 1) acc += s * (cb * lox[(size_t)a]);
-                if (bb == 0) acc -= s * (cb * lox[(size_t)a]);
-            }
-            if (a >= 1 || px) acc -= s * (cb * loy[k]);
-            {
-                const int in = a + 1;
-                if (in < nx) acc += s * (cb * loy[k + 1]);
-                else if (px) acc += s * (cb * loy[(size_t)nx * bb]);
-            }
-            if (!px) {
-                if (a == 1) acc -= s * (cb * loy[(size_t)nx * bb]);
-                if (a == 0) acc += s * (cb * loy[(size_t)nx * bb]);
-            }
-            sLE[e] = acc;
-        }
-    }
     __syncthreads();
 
     // ---- 5: owned faces (as ar_faces): per owned cell its x face, then its y face ------
@@ -446,29 +479,38 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
     }
 }
 
-template <int TX, int TY>
+// FPR fields per round, each round's loads issued one round ahead.  Two fields per
+// round shorten the field loop of a lone 512 x 256 grid by ~10%, but at 182 instead of
+// 146 VGPRs they leave no room for the recompute stream's stage kernels beside it,
+// which costs more in the pipelined sweep: one is the default.
+template <int TX, int TY, int FPR = 1>
 struct RevShapeB {
+    static_assert(FPR == 1 || FPR == 2 || FPR == 3, "6 fields in whole rounds");
+    static constexpr int ROUNDS = 6 / FPR;
+    static constexpr int NT = TX * TY;
     static constexpr int PW = TX + 5, PH = TY + 5, P_SZ = PW * PH;   // primitive: cells [-2, T+2]
     static constexpr int GW = TX + 3, GH = TY + 3, G_SZ = GW * GH;   // slope parts: cells [-1, T+1]
     static constexpr int CW = TX + 1, CH = TY + 1, C_SZ = CW * CH;   // cells [0, T]
-    static constexpr int LDS_DOUBLES = P_SZ + 5 * G_SZ + 2 * C_SZ;
-    static constexpr int CPT = (C_SZ + TX * TY - 1) / (TX * TY);     // cells [0, T] per thread
+    // the round's primitives and slope parts, lam of cell B, lam of the owned cells' U
+    static constexpr int LDS_DOUBLES = FPR * (P_SZ + 5 * G_SZ) + 2 * C_SZ + 4 * NT;
+    static constexpr int CPT = (C_SZ + NT - 1) / NT;                 // cells [0, T] per thread
 };
 
 // lU0_add: the step's lam_u0, added to lam_in at the end (stage 1), else nullptr
-template <int LIMITER, int TX, int TY>
+template <int LIMITER, int TX, int TY, int FPR = 1>
 __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const *U,
                                            const FusedBufs &F, double *const *lin,
                                            const double *const *lU0_add, int tile_x, int tile_y,
                                            double *lds) {
-    using B = RevShapeB<TX, TY>;
-    constexpr int NT = TX * TY;
+    using B = RevShapeB<TX, TY, FPR>;
+    constexpr int NT = B::NT;
     const int tid = threadIdx.x;
     const int nx = G.nx, ny = G.ny, px = G.px, py = G.py;
     const int x0 = tile_x * TX, y0 = tile_y * TY;
-    double *sA = lds;                       // one primitive, cells [-2, T+2]
-    double *sG = sA + B::P_SZ;              // gmx, gpx, gmy, gpy, lP of one field, cells [-1, T+1]
-    double *sB = sG + 5 * B::G_SZ;          // lam of cell B (total), cells [0, T]
+    double *sA = lds;                       // [FPR] primitives, cells [-2, T+2]
+    double *sG = sA + FPR * B::P_SZ;        // [FPR] gmx, gpx, gmy, gpy, lP, cells [-1, T+1]
+    double *sB = sG + FPR * 5 * B::G_SZ;    // lam of cell B (total), cells [0, T]
+    double *sU = sB + 2 * B::C_SZ;          // lam of the owned cells' conserved U (cons_to_prim)
     const double hdx = 0.5 * G.dx, inv_dx = 1.0 / G.dx, inv_2dx = 1.0 / (2.0 * G.dx);
     // valid global cell of a local position, or -1 outside a non-periodic grid
     auto gcell = [&](int lx, int ly, int &gi, int &gj) -> bool {
@@ -481,7 +523,7 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
     double lp6[B::CPT][6];
 
     // per-thread positions of the primitive items and of the slope items, fixed over the
-    // six fields; each field's loads are issued one round ahead of its use
+    // six fields; each round's loads are issued one round ahead of its use
     constexpr int PPT = (B::P_SZ + NT - 1) / NT, GPT = (B::G_SZ + NT - 1) / NT;
     size_t kp[PPT];
 #pragma unroll
@@ -512,143 +554,192 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
         t.cym = !py && j == 0;
         t.cyp = !py && j == ny - 1;
     }
-    double pv[2][PPT], rv[2][GPT][6];
-    auto load = [&](int f, int buf) {
+    double pv[2][FPR][PPT], rv[2][FPR][GPT][6];
+    auto load = [&](int r, int buf) {
 #pragma unroll
-        for (int u = 0; u < PPT; ++u)
-            if (tid + u * NT < B::P_SZ) pv[buf][u] = F.P(f)[kp[u]];
+        for (int h = 0; h < FPR; ++h) {
+            const int f = r * FPR + h;
 #pragma unroll
-        for (int u = 0; u < GPT; ++u) {
-            const GItem &t = gt[u];
-            if (!t.valid) continue;
-            rv[buf][u][0] = F.XL(f)[t.k];
-            rv[buf][u][1] = t.hl ? F.XR(f)[t.kl] : 0.0;   // right state of face i-1/2
-            rv[buf][u][2] = F.YL(f)[t.k];
-            rv[buf][u][3] = t.hb ? F.YR(f)[t.kb] : 0.0;
-            rv[buf][u][4] = F.Pr(f)[t.k];
-            rv[buf][u][5] = F.Pd(f)[t.k];
+            for (int u = 0; u < PPT; ++u)
+                if (tid + u * NT < B::P_SZ) pv[buf][h][u] = F.P(f)[kp[u]];
+#pragma unroll
+            for (int u = 0; u < GPT; ++u) {
+                const GItem &t = gt[u];
+                if (!t.valid) continue;
+                rv[buf][h][u][0] = F.XL(f)[t.k];
+                rv[buf][h][u][1] = t.hl ? F.XR(f)[t.kl] : 0.0;   // right state of face i-1/2
+                rv[buf][h][u][2] = F.YL(f)[t.k];
+                rv[buf][h][u][3] = t.hb ? F.YR(f)[t.kb] : 0.0;
+                rv[buf][h][u][4] = F.Pr(f)[t.k];
+                rv[buf][h][u][5] = (t.cxm || t.cym) ? F.Pd(f)[t.k] : 0.0;   // open low side
+            }
+        }
+    };
+    // the tail's inputs, issued with the last round: per cell of [0, T] its B, lam of its
+    // cell B and its conserved U; for the owned cell of this thread lam_in (and lam_u0)
+    size_t ck[B::CPT];
+    bool cvalid[B::CPT];
+    double tu[B::CPT][8];
+    const int ox = tid % TX, oy = tid / TX;
+    const bool oact = x0 + ox < nx && y0 + oy < ny;
+    const size_t ko = oact ? (size_t)(x0 + ox) + (size_t)nx * (y0 + oy) : 0;
+    double tl[12];
+    auto load_tail = [&]() {
+#pragma unroll
+        for (int u = 0; u < B::CPT; ++u) {
+            const int c = tid + u * NT;
+            int i = 0, j = 0;
+            cvalid[u] = c < B::C_SZ && gcell(c % B::CW, c / B::CW, i, j);
+            ck[u] = (size_t)i + (size_t)nx * j;
+            if (!cvalid[u]) continue;
+            tu[u][0] = F.P(4)[ck[u]];
+            tu[u][1] = F.P(5)[ck[u]];
+            tu[u][2] = F.Bc(0)[ck[u]];
+            tu[u][3] = F.Bc(1)[ck[u]];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) tu[u][4 + q] = U[q][ck[u]];
+        }
+        if (oact) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) tl[q] = lin[q][ko];
+            if (lU0_add) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) tl[6 + q] = lU0_add[q][ko];
+            }
         }
     };
     load(0, 0);
 
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        const int cur = f & 1;
-        // the field's stage primitive over cells [-2, T+2] (clamped like the forward)
+    for (int r = 0; r < B::ROUNDS; ++r) {
+        const int cur = r & 1;
+        // the round's stage primitives over cells [-2, T+2] (clamped like the forward)
 #pragma unroll
-        for (int u = 0; u < PPT; ++u)
-            if (tid + u * NT < B::P_SZ) sA[tid + u * NT] = pv[cur][u];
+        for (int h = 0; h < FPR; ++h)
+#pragma unroll
+            for (int u = 0; u < PPT; ++u)
+                if (tid + u * NT < B::P_SZ) sA[h * B::P_SZ + tid + u * NT] = pv[cur][h][u];
         __syncthreads();
-        if (f < 5) load(f + 1, cur ^ 1);
+        if (r + 1 < B::ROUNDS) load(r + 1, cur ^ 1);
+        else load_tail();
         // slope parts of cells [-1, T+1] (as ar_slopes)
 #pragma unroll
-        for (int u = 0; u < GPT; ++u) {
-            const int idx = tid + u * NT;
-            if (idx >= B::G_SZ) break;
-            const GItem &t = gt[u];
-            if (!t.valid) {
+        for (int h = 0; h < FPR; ++h) {
+            const double *sAh = sA + h * B::P_SZ;
+            double *sGh = sG + h * 5 * B::G_SZ;
 #pragma unroll
-                for (int r = 0; r < 5; ++r) sG[r * B::G_SZ + idx] = 0.0;
-                continue;
+            for (int u = 0; u < GPT; ++u) {
+                const int idx = tid + u * NT;
+                if (idx >= B::G_SZ) break;
+                const GItem &t = gt[u];
+                if (!t.valid) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) sGh[c * B::G_SZ + idx] = 0.0;
+                    continue;
+                }
+                const int lx = idx % B::GW - 1, ly = idx / B::GW - 1;
+                const double *a = sAh + (ly + 2) * B::PW + (lx + 2);
+                const double am = t.cxm ? a[0] : a[-1], ap = t.cxp ? a[0] : a[1];
+                const double bm = t.cym ? a[0] : a[-B::PW], bp = t.cyp ? a[0] : a[B::PW];
+                const double *r6 = rv[cur][h][u];
+                const double xr = r6[1], yr = r6[3];
+                const double lSx = hdx * (r6[0] - xr);
+                const double lSy = hdx * (r6[2] - yr);
+                double lp = r6[0] + xr + r6[2] + yr + r6[4] + r6[5];
+                double dm, d0, dp;
+                if (G.use_lim) limited_slope_grad<LIMITER>(am, a[0], ap, inv_dx, dm, d0, dp);
+                else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
+                double gm = lSx * dm, gp = lSx * dp;
+                lp += lSx * d0;
+                if (t.cxm) { lp += gm; gm = 0.0; }
+                if (t.cxp) { lp += gp; gp = 0.0; }
+                sGh[0 * B::G_SZ + idx] = gm;
+                sGh[1 * B::G_SZ + idx] = gp;
+                if (G.use_lim) limited_slope_grad<LIMITER>(bm, a[0], bp, inv_dx, dm, d0, dp);
+                else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
+                gm = lSy * dm;
+                gp = lSy * dp;
+                lp += lSy * d0;
+                if (t.cym) { lp += gm; gm = 0.0; }
+                if (t.cyp) { lp += gp; gp = 0.0; }
+                sGh[2 * B::G_SZ + idx] = gm;
+                sGh[3 * B::G_SZ + idx] = gp;
+                sGh[4 * B::G_SZ + idx] = lp;
             }
-            const int lx = idx % B::GW - 1, ly = idx / B::GW - 1;
-            const double *a = sA + (ly + 2) * B::PW + (lx + 2);
-            const double am = t.cxm ? a[0] : a[-1], ap = t.cxp ? a[0] : a[1];
-            const double bm = t.cym ? a[0] : a[-B::PW], bp = t.cyp ? a[0] : a[B::PW];
-            const double *r6 = rv[cur][u];
-            const double xr = r6[1], yr = r6[3];
-            const double lSx = hdx * (r6[0] - xr);
-            const double lSy = hdx * (r6[2] - yr);
-            double lp = r6[0] + xr + r6[2] + yr + r6[4] + r6[5];
-            double dm, d0, dp;
-            if (G.use_lim) limited_slope_grad<LIMITER>(am, a[0], ap, inv_dx, dm, d0, dp);
-            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
-            double gm = lSx * dm, gp = lSx * dp;
-            lp += lSx * d0;
-            if (t.cxm) { lp += gm; gm = 0.0; }
-            if (t.cxp) { lp += gp; gp = 0.0; }
-            sG[0 * B::G_SZ + idx] = gm;
-            sG[1 * B::G_SZ + idx] = gp;
-            if (G.use_lim) limited_slope_grad<LIMITER>(bm, a[0], bp, inv_dx, dm, d0, dp);
-            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
-            gm = lSy * dm;
-            gp = lSy * dp;
-            lp += lSy * d0;
-            if (t.cym) { lp += gm; gm = 0.0; }
-            if (t.cyp) { lp += gp; gp = 0.0; }
-            sG[2 * B::G_SZ + idx] = gm;
-            sG[3 * B::G_SZ + idx] = gp;
-            sG[4 * B::G_SZ + idx] = lp;
         }
         __syncthreads();
         // own part plus the neighbours' parts for cells [0, T] (as ar_prims)
 #pragma unroll
-        for (int u = 0; u < B::CPT; ++u) {
-            const int c = tid + u * NT;
-            if (c >= B::C_SZ) break;
-            const int lx = c % B::CW, ly = c / B::CW;
-            int i, j;
-            if (!gcell(lx, ly, i, j)) {
-                lp6[u][f] = 0.0;
-                continue;
+        for (int h = 0; h < FPR; ++h) {
+            const double *sGh = sG + h * 5 * B::G_SZ;
+#pragma unroll
+            for (int u = 0; u < B::CPT; ++u) {
+                const int c = tid + u * NT;
+                if (c >= B::C_SZ) break;
+                const int lx = c % B::CW, ly = c / B::CW;
+                int i, j;
+                if (!gcell(lx, ly, i, j)) {
+                    lp6[u][r * FPR + h] = 0.0;
+                    continue;
+                }
+                const bool hl = i > 0 || px, hr = i < nx - 1 || px;
+                const bool hb = j > 0 || py, ht = j < ny - 1 || py;
+                const int gidx = (ly + 1) * B::GW + (lx + 1);
+                double acc = sGh[4 * B::G_SZ + gidx];
+                if (hl) acc += sGh[1 * B::G_SZ + gidx - 1];
+                if (hr) acc += sGh[0 * B::G_SZ + gidx + 1];
+                if (hb) acc += sGh[3 * B::G_SZ + gidx - B::GW];
+                if (ht) acc += sGh[2 * B::G_SZ + gidx + B::GW];
+                lp6[u][r * FPR + h] = acc;
             }
-            const bool hl = i > 0 || px, hr = i < nx - 1 || px;
-            const bool hb = j > 0 || py, ht = j < ny - 1 || py;
-            const int gidx = (ly + 1) * B::GW + (lx + 1);
-            double acc = sG[4 * B::G_SZ + gidx];
-            if (hl) acc += sG[1 * B::G_SZ + gidx - 1];
-            if (hr) acc += sG[0 * B::G_SZ + gidx + 1];
-            if (hb) acc += sG[3 * B::G_SZ + gidx - B::GW];
-            if (ht) acc += sG[2 * B::G_SZ + gidx + B::GW];
-            lp6[u][f] = acc;
         }
-        __syncthreads();
+        // the next round overwrites sA and sG; the tail uses sB and sU only
+        if (r + 1 < B::ROUNDS) __syncthreads();
     }
-    // reverse of cons_to_prim for cells [0, T]; lam of the conserved input for owned cells
+    // reverse of cons_to_prim for cells [0, T]; lam of the owned cells' U into sU
 #pragma unroll
     for (int u = 0; u < B::CPT; ++u) {
         const int c = tid + u * NT;
         if (c >= B::C_SZ) break;
         const int lx = c % B::CW, ly = c / B::CW;
-        int i, j;
-        if (!gcell(lx, ly, i, j)) {
+        if (!cvalid[u]) {
             sB[c] = 0.0;
             sB[B::C_SZ + c] = 0.0;
             continue;
         }
-        const size_t k = (size_t)i + (size_t)nx * j;
-        const double Bx = F.P(4)[k], By = F.P(5)[k];
         double lU[4] = {0, 0, 0, 0};
-        double lBx = lp6[u][4] + F.Bc(0)[k], lBy = lp6[u][5] + F.Bc(1)[k];
-        cons_to_prim_vjp(U[0][k], U[1][k], U[2][k], U[3][k], Bx, By, G.gamma, G.vol, lp6[u], lU,
-                         lBx, lBy);
+        double lBx = lp6[u][4] + tu[u][2], lBy = lp6[u][5] + tu[u][3];
+        cons_to_prim_vjp(tu[u][4], tu[u][5], tu[u][6], tu[u][7], tu[u][0], tu[u][1], G.gamma,
+                         G.vol, lp6[u], lU, lBx, lBy);
         if (lx < TX && ly < TY && x0 + lx < nx && y0 + ly < ny) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) lin[q][k] += lU[q];
+            for (int q = 0; q < 4; ++q) sU[q * NT + ly * TX + lx] = lU[q];
         }
         sB[c] = lBx;
         sB[B::C_SZ + c] = lBy;
     }
     __syncthreads();
-    // face -> cell B average (as ar_bavg), owned cells
-    {
-        const int lx = tid % TX, ly = tid / TX;
-        const int i = x0 + lx, j = y0 + ly;
-        if (i < nx && j < ny) {
-            const size_t k = (size_t)i + (size_t)nx * j;
-            const int c = ly * B::CW + lx;
-            const double *lbx = sB, *lby = sB + B::C_SZ;
-            double ax = (i == 0 && !px) ? lbx[c] : 0.5 * lbx[c];
-            if (i + 1 < nx || px) ax += 0.5 * lbx[c + 1];
-            double ay = (j == 0 && !py) ? lby[c] : 0.5 * lby[c];
-            if (j + 1 < ny || py) ay += 0.5 * lby[c + B::CW];
-            lin[4][k] += ax;
-            lin[5][k] += ay;
-            if (lU0_add) {
+    // face -> cell B average (as ar_bavg); lam_in of the owned cell written once:
+    // ((lam_in + cell part) + lam_u0)
+    if (oact) {
+        const int i = x0 + ox, j = y0 + oy;
+        const int c = oy * B::CW + ox;
+        const double *lbx = sB, *lby = sB + B::C_SZ;
+        double ax = (i == 0 && !px) ? lbx[c] : 0.5 * lbx[c];
+        if (i + 1 < nx || px) ax += 0.5 * lbx[c + 1];
+        double ay = (j == 0 && !py) ? lby[c] : 0.5 * lby[c];
+        if (j + 1 < ny || py) ay += 0.5 * lby[c + B::CW];
+        double v[6];
 #pragma unroll
-                for (int q = 0; q < 6; ++q) lin[q][k] += lU0_add[q][k];
-            }
+        for (int q = 0; q < 4; ++q) v[q] = tl[q] + sU[q * NT + tid];
+        v[4] = tl[4] + ax;
+        v[5] = tl[5] + ay;
+        if (lU0_add) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v[q] += tl[6 + q];
         }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) lin[q][ko] = v[q];
     }
 }
 

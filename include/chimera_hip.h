@@ -269,6 +269,15 @@ int chimera_adj_step_vjp(chimera_adj_t *adj, const double *const *state_n, doubl
  * first use and kept), then lam <- (dU^{nsteps}/dU^0)^T lam in place (device arrays). */
 int chimera_adj_segment_vjp(chimera_adj_t *adj, const double *const *state0, const double *dt_in,
                             int nsteps, double *const *lam);
+/* The whole reverse sweep of an nsteps run checkpointed every seg_steps steps: states
+ * holds 6 device pointers per checkpoint (checkpoint k = U^{k * seg_steps}, k = 0 ..
+ * ceil(nsteps / seg_steps) - 1), then lam <- (dU^{nsteps}/dU^0)^T lam in place.  The
+ * same bits as chimera_adj_segment_vjp over the segments from last to first; the
+ * recompute of segment k-1 runs on a second stream while segment k is reversed (two
+ * segments of 3 * seg_steps states of device memory).  Replaces the per-segment loop
+ * of the autograd backward; no counterpart in the reference (it has no derivatives). */
+int chimera_adj_sweep(chimera_adj_t *adj, const double *const *states, const double *dt_in,
+                      int nsteps, int seg_steps, double *const *lam);
 void *chimera_adj_stream(chimera_adj_t *adj);
 
 /* Which reverse kernels the adjoint runs: 1 (default) the fused LDS-tiled pair that

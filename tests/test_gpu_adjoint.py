@@ -375,3 +375,33 @@ def test_evolve_batch_gradient_matches_evolve(gpu_lib):
         assert np.array_equal(outs[m].detach().cpu().numpy(), un.detach().cpu().numpy()), m
         assert np.array_equal(U0[m].grad.cpu().numpy(), u.grad.cpu().numpy()), m
     badj.close()
+
+
+@pytest.mark.parametrize("scheme,nsteps,K", [((2, 2, 1, 1), 23, 5), ((3, 1, 1, 1), 7, 3),
+                                             ((1, 2, 1, 1), 6, 10)])
+def test_sweep_matches_segment_loop(gpu_lib, scheme, nsteps, K):
+    """The pipelined sweep (segment k-1 recomputed on a second stream while segment k is
+    reversed) gives the bits of segment_vjp over the segments from last to first, also
+    when the stepper is reused and the last segment is shorter."""
+    torch = _torch()
+    from chimera_amd.adjoint import AdjointStepper
+    p = problems.field_loop(96, 48, scheme=scheme)
+    rng = np.random.default_rng(11)
+    dts = 0.3 * p.dx / 3.0 * rng.uniform(0.6, 1.0, nsteps)
+    x0 = torch.from_numpy(p.state.copy()).cuda()
+    adj = AdjointStepper(p.nx, p.ny, p.dx, p.gamma, p.bc, p.scheme)
+    ckpts, x = [], x0
+    for s in range(nsteps):
+        if s % K == 0:
+            ckpts.append(x.clone())
+        x = adj.forward_step(x, dts[s])
+    lam0 = torch.from_numpy(rng.standard_normal(p.state.shape)).cuda()
+    ref = lam0.clone()
+    for k in reversed(range(len(ckpts))):
+        adj.segment_vjp(ckpts[k], dts[k * K:(k + 1) * K], ref)
+    print(f"sweep {scheme}: {nsteps} steps, {len(ckpts)} segments", flush=True)
+    for _ in range(2):
+        got = adj.sweep(ckpts, dts, K, lam0.clone())
+        assert np.array_equal(got.cpu().numpy(), ref.cpu().numpy())
+    adj.close()
+    assert np.all(np.isfinite(ref.cpu().numpy()))
