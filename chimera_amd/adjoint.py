@@ -211,6 +211,19 @@ class BatchAdjoint:
                                                d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
                                                _ptrs_many(lams)), "chimera_badj_segment_vjp")
 
+    def sweep(self, ckpts, dts, seg_steps: int, lams) -> None:
+        """lams[m] <- (dU_N / dU_0)^T lams[m] in place for a run of dts.shape[1] steps
+        checkpointed every seg_steps steps (ckpts[k][m]: member m at step k * seg_steps),
+        every segment recomputed and reversed, last first, in one call."""
+        d = _dt_table(dts, self.members)
+        nseg = -(-d.shape[1] // seg_steps)
+        if len(ckpts) != nseg or any(len(c) != self.members for c in ckpts):
+            raise ValueError(f"need {nseg} checkpoints of {self.members} members")
+        torch.cuda.synchronize()
+        check(self._L.chimera_badj_sweep(self._h, _ptrs_many([t for c in ckpts for t in c]),
+                                         d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
+                                         int(seg_steps), _ptrs_many(lams)), "chimera_badj_sweep")
+
     def set_tiles(self, shape_a: int = -1, shape_b: int = -1) -> None:
         """Tile shapes of the two reverse passes: 0 = 16x16, 1 = 32x8."""
         check(self._L.chimera_badj_set_tiles(self._h, int(shape_a), int(shape_b)),
@@ -242,12 +255,8 @@ class _EvolveBatch(torch.autograd.Function):
     @staticmethod
     def backward(ctx, *grads):
         ckpts, badj, dts, K = ctx.ckpts, ctx.badj, ctx.dts, ctx.K
-        n = dts.shape[1]
         lams = [g.contiguous().clone() for g in grads]
-        for seg in reversed(range(len(ckpts))):
-            s0 = seg * K
-            s1 = min(s0 + K, n)
-            badj.segment_vjp(ckpts[seg], np.ascontiguousarray(dts[:, s0:s1]), lams)
+        badj.sweep(ckpts, dts, K, lams)
         return (None, None, None, *lams)
 
 

@@ -62,7 +62,7 @@ struct RevTile {
 constexpr int REV_NUM_SHAPES = 4;
 constexpr RevTile REV_SHAPES[REV_NUM_SHAPES] = {{16, 16}, {16, 8}, {8, 8}, {32, 8}};
 
-template <int SOLVER, int LIMITER, bool STORE>
+template <int SOLVER, int LIMITER, int STORE>
 __global__ __launch_bounds__(STAGE_THREADS) void adj_stage_kernel(GridView g, StageStore st,
                                                                  int stage, int use_lim, int mood,
                                                                  int tiles_x, int ntiles) {
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_stage_kernel(GridView g, St
     const int tile = (b % 8) * (nb / 8) + b / 8;
     if (tile >= ntiles) return;
     stage_tile<SOLVER, LIMITER, ATX, ATY, STORE>(g, stage, use_lim != 0, mood != 0, tile % tiles_x,
-                                                 tile / tiles_x, lds, STORE ? &st : nullptr);
+                                                 tile / tiles_x, lds, STORE != STORE_NONE ? &st : nullptr);
 }
 
 // ---- reverse -------------------------------------------------------------------
@@ -339,6 +339,7 @@ __global__ void ar_bavg(AdjGrid G, RevBufs R, double *const *lin) {
 
 template <int SOLVER, int LIMITER, int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, FusedBufs F,
+                                                   const double *const *Up,
                                                    const double *const *lout, double *const *lin,
                                                    double *const *lU0, int stage, double dt,
                                                    int tiles_x, int ntiles) {
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, Fused
     const int b = blockIdx.x, nb = gridDim.x;   // nb is a multiple of 8
     const int tile = (b % 8) * (nb / 8) + b / 8;
     if (tile >= ntiles) return;
-    rev_tile_a<SOLVER, LIMITER, TX, TY>(g, G, F, lout, lin, lU0, stage, dt, tile % tiles_x,
+    rev_tile_a<SOLVER, LIMITER, TX, TY>(g, G, F, Up, lout, lin, lU0, stage, dt, tile % tiles_x,
                                         tile / tiles_x, lds);
 }
 
@@ -430,10 +431,13 @@ struct chimera_adj {
     // dt of the steps being processed (device; the stage kernel reads *dt)
     double *dt_dev = nullptr;
     int dt_cap = 0;
-    // checkpoint recomputation: the recomputed stage inputs
+    // checkpoint recomputation: the recomputed stage inputs and each stage's Up
     double *traj = nullptr;       // traj_cap x 6 fields
     double **traj_tab = nullptr;  // device tables, 6 pointers per slot
     int traj_cap = 0;
+    double *uptraj = nullptr;     // up_cap x 4 fields
+    double **up_tab = nullptr;    // device tables, 4 pointers per slot
+    int up_cap = 0;
     // the whole-run sweep: segment k-1 is recomputed on rstream while segment k is
     // reversed on stream (two trajectory halves, alternating by segment parity)
     hipStream_t rstream = nullptr;
@@ -446,9 +450,10 @@ namespace {
 
 // device pointer tables: [0..2] stage_state, [3] lam_a, [4] lam_b, [5] lam_u0,
 // [6] scratch_out, [7] caller input, [8] caller output, [9] caller lam in,
-// [10] caller lam out, [11] lam_c
+// [10] caller lam out, [11] lam_c, [12..14] Up of stages 1..3 (S[].Up, 4 pointers)
 enum { TB_S0 = 0, TB_S1 = 1, TB_S2 = 2, TB_LA = 3, TB_LB = 4, TB_LU0 = 5, TB_SCR = 6, TB_IN = 7,
-       TB_OUT = 8, TB_LIN = 9, TB_LOUT = 10, TB_LC = 11, TB_COUNT = 12 };
+       TB_OUT = 8, TB_LIN = 9, TB_LOUT = 10, TB_LC = 11, TB_UP1 = 12, TB_UP2 = 13, TB_UP3 = 14,
+       TB_COUNT = 15 };
 
 using Tab = double *const *;
 
@@ -484,15 +489,20 @@ int set_dts(chimera_adj *a, const double *dts, int k) {
 
 using AdjStageFn = void (*)(GridView, StageStore, int, int, int, int, int);
 
-AdjStageFn adj_stage_fn(int solver, int limiter, bool store) {
-    static const AdjStageFn t[2][3][2] = {
-        {{adj_stage_kernel<RUSANOV, LIM_VAN_LEER, false>, adj_stage_kernel<RUSANOV, LIM_MC, false>},
-         {adj_stage_kernel<HLLE, LIM_VAN_LEER, false>, adj_stage_kernel<HLLE, LIM_MC, false>},
-         {adj_stage_kernel<HLLD, LIM_VAN_LEER, false>, adj_stage_kernel<HLLD, LIM_MC, false>}},
-        {{adj_stage_kernel<RUSANOV, LIM_VAN_LEER, true>, adj_stage_kernel<RUSANOV, LIM_MC, true>},
-         {adj_stage_kernel<HLLE, LIM_VAN_LEER, true>, adj_stage_kernel<HLLE, LIM_MC, true>},
-         {adj_stage_kernel<HLLD, LIM_VAN_LEER, true>, adj_stage_kernel<HLLD, LIM_MC, true>}}};
-    return t[store ? 1 : 0][solver - 1][limiter - 1];
+template <int MODE>
+constexpr AdjStageFn adj_stage_of(int solver, int limiter) {
+    return solver == RUSANOV ? (limiter == LIM_MC ? adj_stage_kernel<RUSANOV, LIM_MC, MODE>
+                                                  : adj_stage_kernel<RUSANOV, LIM_VAN_LEER, MODE>)
+         : solver == HLLE    ? (limiter == LIM_MC ? adj_stage_kernel<HLLE, LIM_MC, MODE>
+                                                  : adj_stage_kernel<HLLE, LIM_VAN_LEER, MODE>)
+                             : (limiter == LIM_MC ? adj_stage_kernel<HLLD, LIM_MC, MODE>
+                                                  : adj_stage_kernel<HLLD, LIM_VAN_LEER, MODE>);
+}
+
+AdjStageFn adj_stage_fn(int solver, int limiter, int mode) {
+    return mode == STORE_ALL  ? adj_stage_of<STORE_ALL>(solver, limiter)
+         : mode == STORE_UP   ? adj_stage_of<STORE_UP>(solver, limiter)
+                              : adj_stage_of<STORE_NONE>(solver, limiter);
 }
 
 GridView grid_view(const chimera_adj *a, const Ptr6 &in, const Ptr6 &u0, const Ptr6 &out,
@@ -515,19 +525,25 @@ GridView grid_view(const chimera_adj *a, const Ptr6 &in, const Ptr6 &u0, const P
     return g;
 }
 
-// One stage through the fused kernel; with store, keeping the intermediates in
-// S[stage - 1].
+// One stage through the fused kernel on stream st (default: the adjoint's stream).
+// STORE_ALL keeps the intermediates in S[stage - 1]; STORE_UP keeps only Up, in the
+// four arrays up (host pointer set).
 int run_forward_stage(chimera_adj *a, int stage, const Ptr6 &in, const Ptr6 &u0, const Ptr6 &out,
-                      const double *dt_dev, bool store = true, hipStream_t st = nullptr) {
+                      const double *dt_dev, int mode = STORE_ALL, hipStream_t st = nullptr,
+                      double *const *up = nullptr) {
     GridView g = grid_view(a, in, u0, out, dt_dev);
-    hipLaunchKernelGGL(adj_stage_fn(a->sch.riemann_solver, a->sch.slope_limiter, store),
-                       dim3(a->grid), dim3(STAGE_THREADS), 0, st ? st : a->stream, g, a->S[stage - 1], stage,
+    StageStore store = a->S[stage - 1];
+    if (mode == STORE_UP)
+        for (int q = 0; q < 4; ++q) store.Up[q] = up[q];
+    hipLaunchKernelGGL(adj_stage_fn(a->sch.riemann_solver, a->sch.slope_limiter, mode),
+                       dim3(a->grid), dim3(STAGE_THREADS), 0, st ? st : a->stream, g, store, stage,
                        a->G.use_lim, a->G.mood, a->tiles_x, a->ntiles);
     return launch_check("adjoint forward stage");
 }
 
-using TileAFn = void (*)(GridView, AdjGrid, FusedBufs, const double *const *, double *const *,
-                         double *const *, int, double, int, int);
+using TileAFn = void (*)(GridView, AdjGrid, FusedBufs, const double *const *,
+                         const double *const *, double *const *, double *const *, int, double, int,
+                         int);
 using TileBFn = void (*)(AdjGrid, const double *const *, FusedBufs, double *const *,
                          const double *const *, int, int);
 
@@ -569,17 +585,17 @@ void rev_launch_shape(const chimera_adj *a, int shape, int &tiles_x, int &ntiles
 }
 
 // The reverse of one stage through the fused kernels: in is the stage input (host
-// pointer set and device table tin), u0 the step input.  Stage 3 starts lam_u0,
-// stage 2 adds to it and stage 1 adds it to lam_in.
+// pointer set and device table tin), u0 the step input, tup the stage's Up (device
+// table of 4).  Stage 3 starts lam_u0, stage 2 adds to it and stage 1 adds it to lam_in.
 int run_reverse_stage_fused(chimera_adj *a, int stage, const Ptr6 &in, Tab tin, const Ptr6 &u0,
-                            Tab tlout, Tab tlin, double dt) {
+                            Tab tup, Tab tlout, Tab tlin, double dt) {
     GridView g = grid_view(a, in, u0, in, nullptr);
     int tx, nt, grid;
     rev_launch_shape(a, a->rshape_a, tx, nt, grid);
     const RevTile ta = REV_SHAPES[a->rshape_a];
     hipLaunchKernelGGL(tile_a_fn(a->sch.riemann_solver, a->sch.slope_limiter, a->rshape_a),
-                       dim3(grid), dim3(ta.tx * ta.ty), 0, a->stream, g, a->G, a->F, tlout, tlin,
-                       table(a, TB_LU0), stage, dt, tx, nt);
+                       dim3(grid), dim3(ta.tx * ta.ty), 0, a->stream, g, a->G, a->F, tup, tlout,
+                       tlin, table(a, TB_LU0), stage, dt, tx, nt);
     rev_launch_shape(a, a->rshape_b, tx, nt, grid);
     const RevTile tb = REV_SHAPES[a->rshape_b];
     hipLaunchKernelGGL(tile_b_fn(a->sch.slope_limiter, a->rshape_b), dim3(grid),
@@ -613,26 +629,36 @@ int run_reverse_stage(chimera_adj *a, int stage, Tab tin, Tab tu0, Tab tlout, Ta
 // lam_in = (dU^{n+1}/dU^n)^T lam_out for one step at U^n (host pointers hs, device
 // table ts), dt = *dt_dev; asynchronous.  tlout and tlin must differ; neither may be
 // LA, LB or LU0.
-// The stage inputs U^(1), U^(2) of the step may be supplied (host pointer sets s1, s2
-// and device tables t1, t2, e.g. recorded by the segment recompute); otherwise the
-// fused path computes them with two forward stages.
+// The stage inputs U^(1), U^(2) of the step and the Up of its three stages may be
+// supplied (host pointer sets s1, s2, device tables t1, t2 and up[3], as the segment
+// recompute records them); otherwise the fused path computes them with three forward
+// stages.
 int vjp_core(chimera_adj *a, const Ptr6 &hs, Tab ts, const double *dt_dev, double dt, Tab tlout,
              Tab tlin, const Ptr6 *s1 = nullptr, Tab t1 = nullptr, const Ptr6 *s2 = nullptr,
-             Tab t2 = nullptr) {
+             Tab t2 = nullptr, const Tab *up = nullptr) {
     const int nb = nblocks(a->n);
     if (a->fused) {
-        // the reverse kernels recompute everything else from the stage inputs
-        if (!s1 || !s2) {
-            CH_TRY(run_forward_stage(a, 1, hs, hs, a->stage_state[1], dt_dev, false));
-            CH_TRY(run_forward_stage(a, 2, a->stage_state[1], hs, a->stage_state[2], dt_dev, false));
+        // the reverse kernels recompute everything else from the stage inputs and Up
+        Tab ups[3];
+        if (!s1 || !s2 || !up) {
+            CH_TRY(run_forward_stage(a, 1, hs, hs, a->stage_state[1], dt_dev, STORE_UP, nullptr,
+                                     a->S[0].Up));
+            CH_TRY(run_forward_stage(a, 2, a->stage_state[1], hs, a->stage_state[2], dt_dev,
+                                     STORE_UP, nullptr, a->S[1].Up));
+            CH_TRY(run_forward_stage(a, 3, a->stage_state[2], hs, a->scratch_out, dt_dev, STORE_UP,
+                                     nullptr, a->S[2].Up));
             s1 = &a->stage_state[1];
             s2 = &a->stage_state[2];
             t1 = table(a, TB_S1);
             t2 = table(a, TB_S2);
+            for (int k = 0; k < 3; ++k) ups[k] = table(a, TB_UP1 + k);
+        } else {
+            for (int k = 0; k < 3; ++k) ups[k] = up[k];
         }
-        CH_TRY(run_reverse_stage_fused(a, 3, *s2, t2, hs, tlout, table(a, TB_LB), dt));
-        CH_TRY(run_reverse_stage_fused(a, 2, *s1, t1, hs, table(a, TB_LB), table(a, TB_LA), dt));
-        return run_reverse_stage_fused(a, 1, hs, ts, hs, table(a, TB_LA), tlin, dt);
+        CH_TRY(run_reverse_stage_fused(a, 3, *s2, t2, hs, ups[2], tlout, table(a, TB_LB), dt));
+        CH_TRY(run_reverse_stage_fused(a, 2, *s1, t1, hs, ups[1], table(a, TB_LB), table(a, TB_LA),
+                                       dt));
+        return run_reverse_stage_fused(a, 1, hs, ts, hs, ups[0], table(a, TB_LA), tlin, dt);
     }
     // forward: keep every stage's intermediates
     CH_TRY(run_forward_stage(a, 1, hs, hs, a->stage_state[1], dt_dev));
@@ -648,24 +674,35 @@ int vjp_core(chimera_adj *a, const Ptr6 &hs, Tab ts, const double *dt_dev, doubl
     return launch_check("adjoint accumulate");
 }
 
-int ensure_traj(chimera_adj *a, int slots) {
-    if (slots <= a->traj_cap) return CHIMERA_OK;
+// slots x fields arrays of n cells in one allocation, with a device table of fields
+// pointers per slot (grown on demand, kept)
+int ensure_slots(chimera_adj *a, double *&buf, double **&tab, int &cap, int slots, int fields) {
+    if (slots <= cap) return CHIMERA_OK;
     CH_HIP(hipStreamSynchronize(a->stream));
-    (void)hipFree(a->traj);
-    (void)hipFree(a->traj_tab);
-    a->traj = nullptr;
-    a->traj_tab = nullptr;
-    a->traj_cap = 0;
-    if (hipMalloc(&a->traj, (size_t)slots * 6 * a->n * sizeof(double)) != hipSuccess ||
-        hipMalloc(&a->traj_tab, (size_t)slots * 6 * sizeof(double *)) != hipSuccess) {
-        set_error("adjoint trajectory storage (%d states of %zu cells) failed", slots, a->n);
+    if (a->rstream) CH_HIP(hipStreamSynchronize(a->rstream));
+    (void)hipFree(buf);
+    (void)hipFree(tab);
+    buf = nullptr;
+    tab = nullptr;
+    cap = 0;
+    if (hipMalloc(&buf, (size_t)slots * fields * a->n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&tab, (size_t)slots * fields * sizeof(double *)) != hipSuccess) {
+        set_error("adjoint trajectory storage (%d x %d arrays of %zu cells) failed", slots, fields,
+                  a->n);
         return CHIMERA_ENOMEM;
     }
-    std::vector<double *> h((size_t)slots * 6);
-    for (size_t k = 0; k < h.size(); ++k) h[k] = a->traj + k * a->n;
-    CH_HIP(hipMemcpy(a->traj_tab, h.data(), h.size() * sizeof(double *), hipMemcpyHostToDevice));
-    a->traj_cap = slots;
+    std::vector<double *> h((size_t)slots * fields);
+    for (size_t k = 0; k < h.size(); ++k) h[k] = buf + k * a->n;
+    CH_HIP(hipMemcpy(tab, h.data(), h.size() * sizeof(double *), hipMemcpyHostToDevice));
+    cap = slots;
     return CHIMERA_OK;
+}
+
+// trajectories of `halves` segments of K steps: per half 3K + 1 states (the stage
+// inputs and the last stage's output) and 3K Up sets
+int ensure_traj(chimera_adj *a, int K, int halves) {
+    CH_TRY(ensure_slots(a, a->traj, a->traj_tab, a->traj_cap, halves * (3 * K + 1), 6));
+    return ensure_slots(a, a->uptraj, a->up_tab, a->up_cap, halves * 3 * K, 4);
 }
 
 Ptr6 traj_slot(chimera_adj *a, int s) {
@@ -676,20 +713,33 @@ Ptr6 traj_slot(chimera_adj *a, int s) {
 
 Tab slot_tab(chimera_adj *a, int s) { return (Tab)(a->traj_tab + 6 * (size_t)s); }
 
+// Up slot s: four host pointers / its device table
+struct Ptr4 {
+    double *p[4];
+};
+Ptr4 up_slot(chimera_adj *a, int s) {
+    Ptr4 r;
+    for (int q = 0; q < 4; ++q) r.p[q] = a->uptraj + (4 * (size_t)s + q) * a->n;
+    return r;
+}
+Tab up_tab(chimera_adj *a, int s) { return (Tab)(a->up_tab + 4 * (size_t)s); }
+
 // Recompute one checkpoint segment from h0 on stream st, keeping every stage input in
-// the trajectory slots from base on: slot base+3s holds U^s (s >= 1; U^0 is h0), slots
-// base+3s+1 and base+3s+2 the stage outputs U^(1), U^(2).  dt_dev: the segment's dts.
+// the trajectory slots from base on -- slot base+3s holds U^s (s >= 1; U^0 is h0),
+// slots base+3s+1 and base+3s+2 the stage outputs U^(1), U^(2), slot base+3K the last
+// step's output -- and the Up of step s, stage st in Up slot upbase+3s+st-1.  dt_dev:
+// the segment's dts.
 int recompute_segment(chimera_adj *a, hipStream_t st, const Ptr6 &h0, const double *dt_dev,
-                      int nsteps, int base) {
+                      int nsteps, int base, int upbase) {
     for (int s = 0; s < nsteps; ++s) {
         const Ptr6 in = s == 0 ? h0 : traj_slot(a, base + 3 * s);
         const double *dts = dt_dev + s;
-        CH_TRY(run_forward_stage(a, 1, in, in, traj_slot(a, base + 3 * s + 1), dts, false, st));
-        CH_TRY(run_forward_stage(a, 2, traj_slot(a, base + 3 * s + 1), in,
-                                 traj_slot(a, base + 3 * s + 2), dts, false, st));
-        if (s + 1 < nsteps)
-            CH_TRY(run_forward_stage(a, 3, traj_slot(a, base + 3 * s + 2), in,
-                                     traj_slot(a, base + 3 * s + 3), dts, false, st));
+        for (int stage = 1; stage <= 3; ++stage) {
+            const Ptr6 sin = stage == 1 ? in : traj_slot(a, base + 3 * s + stage - 1);
+            const Ptr4 up = up_slot(a, upbase + 3 * s + stage - 1);
+            CH_TRY(run_forward_stage(a, stage, sin, in, traj_slot(a, base + 3 * s + stage), dts,
+                                     STORE_UP, st, up.p));
+        }
     }
     return CHIMERA_OK;
 }
@@ -698,13 +748,16 @@ int recompute_segment(chimera_adj *a, hipStream_t st, const Ptr6 &h0, const doub
 // set and device table).  lam ping-pongs between cur and other; on return cur holds
 // lam at the segment's start.
 int reverse_segment(chimera_adj *a, const Ptr6 &h0, Tab t0, const double *dt_dev,
-                    const double *dt_host, int nsteps, int base, Tab &cur, Tab &other) {
+                    const double *dt_host, int nsteps, int base, int upbase, Tab &cur,
+                    Tab &other) {
     for (int s = nsteps - 1; s >= 0; --s) {
         const Ptr6 hs = s == 0 ? h0 : traj_slot(a, base + 3 * s);
         Tab ts = s == 0 ? t0 : slot_tab(a, base + 3 * s);
         const Ptr6 s1 = traj_slot(a, base + 3 * s + 1), s2 = traj_slot(a, base + 3 * s + 2);
+        const Tab up[3] = {up_tab(a, upbase + 3 * s), up_tab(a, upbase + 3 * s + 1),
+                           up_tab(a, upbase + 3 * s + 2)};
         CH_TRY(vjp_core(a, hs, ts, dt_dev + s, dt_host[s], cur, other, &s1,
-                        slot_tab(a, base + 3 * s + 1), &s2, slot_tab(a, base + 3 * s + 2)));
+                        slot_tab(a, base + 3 * s + 1), &s2, slot_tab(a, base + 3 * s + 2), up));
         Tab t = cur;
         cur = other;
         other = t;
@@ -752,6 +805,8 @@ extern "C" int chimera_adj_destroy(chimera_adj_t *a) {
     (void)hipFree(a->dt_dev);
     (void)hipFree(a->traj);
     (void)hipFree(a->traj_tab);
+    (void)hipFree(a->uptraj);
+    (void)hipFree(a->up_tab);
     (void)hipFree(a->ck_tab);
     if (a->rstream) {
         (void)hipStreamSynchronize(a->rstream);
@@ -850,6 +905,11 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
     if (rc == CHIMERA_OK) rc = set_table(a, TB_LC, a->lam_c.p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_LU0, a->lam_u0.p);
     if (rc == CHIMERA_OK) rc = set_table(a, TB_SCR, a->scratch_out.p);
+    for (int st = 0; st < 3 && rc == CHIMERA_OK; ++st) {
+        const double *up[6] = {a->S[st].Up[0], a->S[st].Up[1], a->S[st].Up[2], a->S[st].Up[3],
+                               nullptr, nullptr};
+        rc = set_table(a, TB_UP1 + st, up);
+    }
     if (rc == CHIMERA_OK && hipStreamSynchronize(a->stream) != hipSuccess) rc = CHIMERA_EHIP;
     if (rc != CHIMERA_OK) {
         chimera_adj_destroy(a);
@@ -902,15 +962,15 @@ extern "C" int chimera_adj_segment_vjp(chimera_adj_t *a, const double *const *st
     CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
     for (int s = 0; s < nsteps; ++s)
         CH_REQUIRE(dt_in[s] > 0.0, "dt_in[%d] = %g is not positive", s, dt_in[s]);
-    CH_TRY(ensure_traj(a, 3 * nsteps));
+    CH_TRY(ensure_traj(a, nsteps, 1));
     CH_TRY(set_dts(a, dt_in, nsteps));
     CH_TRY(set_table(a, TB_IN, state0));
     CH_TRY(set_table(a, TB_LIN, (const double *const *)lam));
     const Ptr6 h0 = ptr6(state0);
-    CH_TRY(recompute_segment(a, a->stream, h0, a->dt_dev, nsteps, 0));
+    CH_TRY(recompute_segment(a, a->stream, h0, a->dt_dev, nsteps, 0, 0));
     // lam ping-pongs between the caller's arrays and lam_c
     Tab cur = table(a, TB_LIN), other = table(a, TB_LC);
-    CH_TRY(reverse_segment(a, h0, table(a, TB_IN), a->dt_dev, dt_in, nsteps, 0, cur, other));
+    CH_TRY(reverse_segment(a, h0, table(a, TB_IN), a->dt_dev, dt_in, nsteps, 0, 0, cur, other));
     CH_TRY(finish_lam(a, cur, lam));
     CH_HIP(hipStreamSynchronize(a->stream));
     return CHIMERA_OK;
@@ -926,7 +986,7 @@ extern "C" int chimera_adj_sweep(chimera_adj_t *a, const double *const *states, 
     for (int k = 0; k < 6 * nseg; ++k) CH_REQUIRE(states[k], "states[%d] is NULL", k);
     const int K = std::min(seg_steps, nsteps);
     // one segment: the trajectory halves alternate, so two are enough
-    CH_TRY(ensure_traj(a, (nseg > 1 ? 2 : 1) * 3 * K));
+    CH_TRY(ensure_traj(a, K, nseg > 1 ? 2 : 1));
     CH_TRY(ensure_sweep(a, nseg));
     CH_TRY(set_dts(a, dt_in, nsteps));
     CH_TRY(set_table(a, TB_LIN, (const double *const *)lam));
@@ -938,7 +998,8 @@ extern "C" int chimera_adj_sweep(chimera_adj_t *a, const double *const *states, 
     auto recompute = [&](int k) -> int {
         const int b = k & 1;
         CH_TRY(recompute_segment(a, a->rstream, ptr6(states + 6 * (size_t)k),
-                                 a->dt_dev + (size_t)k * seg_steps, seg_len(k), b * 3 * K));
+                                 a->dt_dev + (size_t)k * seg_steps, seg_len(k), b * (3 * K + 1),
+                                 b * 3 * K));
         CH_HIP(hipEventRecord(a->ev_traj[b], a->rstream));
         return CHIMERA_OK;
     };
@@ -953,7 +1014,7 @@ extern "C" int chimera_adj_sweep(chimera_adj_t *a, const double *const *states, 
         CH_HIP(hipStreamWaitEvent(a->stream, a->ev_traj[k & 1], 0));
         CH_TRY(reverse_segment(a, ptr6(states + 6 * (size_t)k), (Tab)(a->ck_tab + 6 * (size_t)k),
                                a->dt_dev + (size_t)k * seg_steps, dt_in + (size_t)k * seg_steps,
-                               seg_len(k), (k & 1) * 3 * K, cur, other));
+                               seg_len(k), (k & 1) * (3 * K + 1), (k & 1) * 3 * K, cur, other));
         CH_HIP(hipEventRecord(a->ev_rev[k & 1], a->stream));
     }
     CH_TRY(finish_lam(a, cur, lam));

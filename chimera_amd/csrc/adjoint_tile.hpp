@@ -65,17 +65,18 @@ __device__ __forceinline__ void face_state_reverse(double lam[6], int s, bool to
 
 // ---- fused reverse -----------------------------------------------------------------
 //
-// The reverse in two LDS-tiled passes that recompute the forward stage instead of
+// The reverse in two LDS-tiled passes that recompute the forward stage -- all of it
+// but the flux update, whose result Up the segment recompute keeps -- instead of
 // reading stored intermediates (the stored-intermediate kernels ar_* of adjoint.hip are
 // the cross-check).
 //
 // rev_tile_a (one TX x TY tile of owned cells per workgroup) replaces the STORE
 // recompute, ar_cell_tail, ar_corner and ar_faces: it loads the primitives of cells
 // [-2, TX+2] x [-2, TY+2] together with lam of -Ez at the corners around the owned
-// faces (one load latency for both), computes the fluxes of the faces of cells
-// [0, TX] x [0, TY] (forward phase 2), the flux update, floors, outflow correction and
-// blend of those cells with the reverse of the tail (lam of Up into LDS), and finally,
-// per owned cell, the face-state recompute, the
+// faces (one load latency for both), takes Up of cells [0, TX] x [0, TY] (the state
+// after the flux update, before the floors) as the segment recompute stored it,
+// recomputes their floors, outflow correction and blend with the reverse of that tail
+// (lam of Up into LDS), and finally, per owned cell, the face-state recompute, the
 // Riemann VJP and the reverse of the face construction of its x and y faces.  It
 // writes what the slope / primitive reverse needs (stage primitives, lam of the
 // face-state candidates, lam of the cell resets, of the outflow primitives and of the
@@ -91,11 +92,9 @@ __device__ __forceinline__ void face_state_reverse(double lam[6], int s, bool to
 template <int TX, int TY>
 struct RevShapeA {
     static constexpr int PW = TX + 5, PH = TY + 5, P_SZ = PW * PH;        // cells [-2, T+2]
-    static constexpr int FXW = TX + 2, FXH = TY + 1, FX_SZ = FXW * FXH;   // x faces: left cell [-1, TX] x [0, TY]
-    static constexpr int FYW = TX + 1, FYH = TY + 2, FY_SZ = FYW * FYH;   // y faces: lower cell [0, TX] x [-1, TY]
     static constexpr int CW = TX + 1, CH = TY + 1, C_SZ = CW * CH;        // lam Up: cells [0, TX] x [0, TY]
     static constexpr int EW = TX + 1, EH = TY + 1, E_SZ = EW * EH;        // lam -Ez: corners [-1, TX) x [-1, TY)
-    static constexpr int LDS_DOUBLES = 6 * P_SZ + 4 * FX_SZ + 4 * FY_SZ + 4 * C_SZ + E_SZ;
+    static constexpr int LDS_DOUBLES = 6 * P_SZ + 4 * C_SZ + E_SZ;
 };
 
 // Face states of one face from the primitives in LDS exactly as the forward stage
@@ -131,31 +130,29 @@ __device__ __forceinline__ unsigned long long face_pair(const double *b0, int ps
 
 template <int SOLVER, int LIMITER, int TX, int TY>
 __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, const FusedBufs &F,
-                                           const double *const *lout, double *const *lin,
-                                           double *const *lU0, int stage, double dt, int tile_x,
-                                           int tile_y, double *lds) {
+                                           const double *const *Up, const double *const *lout,
+                                           double *const *lin, double *const *lU0, int stage,
+                                           double dt, int tile_x, int tile_y, double *lds) {
     using A = RevShapeA<TX, TY>;
     constexpr int NT = TX * TY;
     const int tid = threadIdx.x;
     const int nx = g.nx, ny = g.ny, px = g.px, py = g.py;
     const int x0 = tile_x * TX, y0 = tile_y * TY;
     double *sP = lds;                          // [6][PH][PW]
-    double *sFX = sP + 6 * A::P_SZ;            // [4][FXH][FXW] mass, Momx, Momy, energy
-    double *sFY = sFX + 4 * A::FX_SZ;          // [4][FYH][FYW]
-    double *sLU = sFY + 4 * A::FY_SZ;          // [4][CH][CW]
+    double *sLU = sP + 6 * A::P_SZ;            // [4][CH][CW]
     double *sLE = sLU + 4 * A::C_SZ;           // [EH][EW]
     const double dx = g.dx, vol = g.vol, gam = g.gamma;
     const double inv_dx = 1.0 / dx, inv_2dx = 1.0 / (2.0 * dx), hdx = 0.5 * dx;
     const bool use_lim = G.use_lim != 0, mood = G.mood != 0;
 
-    // loads of phase 3 (the stage input, u^n and lam_out of cells [0, TX] x [0, TY]),
-    // issued first so that they arrive while phases 1 and 2 compute
+    // loads of phase 3 (Up as the forward stage left it, u^n and lam_out of cells
+    // [0, TX] x [0, TY]), issued first so that they arrive while phase 1 computes
     constexpr int CPT = (A::C_SZ + NT - 1) / NT;
     const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
     const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
     size_t ck[CPT];
     bool cvalid[CPT];
-    double cin[CPT][4], cu0[CPT][4], clo[CPT][6];
+    double cup[CPT][4], cu0[CPT][4], clo[CPT][6];
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
         const int c = tid + u * NT;
@@ -165,7 +162,7 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
         if (cvalid[u]) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                cin[u][q] = g.uin[q][ck[u]];
+                cup[u][q] = Up[q][ck[u]];
                 cu0[u][q] = stage > 1 ? g.u0[q][ck[u]] : 0.0;
             }
 #pragma unroll
@@ -272,38 +269,6 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
     }
     __syncthreads();
 
-    // ---- 2: fluxes of the faces of cells [0, TX] x [0, TY] ---------------------------
-    for (int f = tid; f < A::FX_SZ + A::FY_SZ; f += NT) {
-        const int dir = f >= A::FX_SZ;
-        int i, j;
-        if (!dir) {
-            i = f % A::FXW - 1;
-            j = f / A::FXW;
-        } else {
-            i = (f - A::FX_SZ) % A::FYW;
-            j = (f - A::FX_SZ) / A::FYW - 1;
-        }
-        const int lpos = (j + 2) * A::PW + (i + 2);
-        const bool last = dir ? (!py && map_index(y0 + j, ny, py) == ny - 1)
-                              : (!px && map_index(x0 + i, nx, px) == nx - 1);
-        double L[6], Rs[6];
-        (void)face_pair<LIMITER>(sP + lpos, A::P_SZ, dir ? A::PW : 1, dir ? 1 : A::PW, last, mood,
-                                 use_lim, inv_dx, inv_2dx, hdx, dir, L, Rs);
-        FaceState fa{L[F_RHO], dir ? L[F_VY] : L[F_VX], dir ? L[F_VX] : L[F_VY], L[F_P],
-                     dir ? L[F_BY] : L[F_BX], dir ? L[F_BX] : L[F_BY]};
-        FaceState fb{Rs[F_RHO], dir ? Rs[F_VY] : Rs[F_VX], dir ? Rs[F_VX] : Rs[F_VY], Rs[F_P],
-                     dir ? Rs[F_BY] : Rs[F_BX], dir ? Rs[F_BX] : Rs[F_BY]};
-        const Flux5 fl = riemann<SOLVER>(fa, fb, gam);
-        double *sF = dir ? sFY : sFX;
-        const int fsz = dir ? A::FY_SZ : A::FX_SZ;
-        const int idx = dir ? f - A::FX_SZ : f;
-        sF[0 * fsz + idx] = fl.mass;
-        sF[1 * fsz + idx] = dir ? fl.mom_t : fl.mom_n;
-        sF[2 * fsz + idx] = dir ? fl.mom_n : fl.mom_t;
-        sF[3 * fsz + idx] = fl.energy;
-    }
-    __syncthreads();
-
     // ---- 3: forward tail and its reverse for cells [0, TX] x [0, TY] ------------------
     const double dtdx = dt * dx;
 #pragma unroll
@@ -319,28 +284,15 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
         }
         const int gi = map_index(gx, nx, px), gj = map_index(gy, ny, py);
         const size_t k = ck[u];
-        const int xr = cj * A::FXW + (ci + 1);         // x face right of the cell
-        const int yr = (cj + 1) * A::FYW + ci;         // y face above the cell
         const bool xlo_open = (gi == 0) && !px;
         const bool ylo_open = (gj == 0) && !py;
-        double Up[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            double v = cin[u][q];
-            const double fr = sFX[q * A::FX_SZ + xr];
-            if (xlo_open) v = v - dtdx * fr;
-            else v = v - dtdx * fr + dtdx * sFX[q * A::FX_SZ + xr - 1];
-            const double ft = sFY[q * A::FY_SZ + yr];
-            if (ylo_open) v = v - dtdx * ft;
-            else v = v - dtdx * ft + dtdx * sFY[q * A::FY_SZ + yr - A::FYW];
-            Up[q] = v;
-        }
+        const double *const Upc = cup[u];
         const int pc = (cj + 2) * A::PW + (ci + 2);
         double P6[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) P6[q] = sP[q * A::P_SZ + pc];
         const double Bx = P6[4], By = P6[5];
-        double U3[4] = {Up[0], Up[1], Up[2], Up[3]};
+        double U3[4] = {Upc[0], Upc[1], Upc[2], Upc[3]};
         conserved_floors(U3[0], U3[1], U3[2], U3[3], Bx, By, vol, gam);
         if (xlo_open || ylo_open) {
             const double rho = P6[0], vx = P6[1], vy = P6[2], P = P6[3];
@@ -398,7 +350,7 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
         if (xlo_open) outflow_vjp(0, P6, gam, dtdx, l3, lPd);
         if (ylo_open) outflow_vjp(1, P6, gam, dtdx, l3, lPd);
         double lUp[4];
-        floors_vjp(Up, Bx, By, vol, gam, l3, lUp, lBx, lBy);
+        floors_vjp(Upc, Bx, By, vol, gam, l3, lUp, lBx, lBy);
 #pragma unroll
         for (int q = 0; q < 4; ++q) sLU[q * A::C_SZ + c] = lUp[q];
         if (owned) {

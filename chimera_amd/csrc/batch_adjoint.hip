@@ -8,7 +8,10 @@
 // then per step, newest first, runs the fused reverse pair (rev_tile_a / rev_tile_b of
 // adjoint_tile.hpp) for stages 3, 2, 1 over all members; each member's lam is updated in
 // place.  The arithmetic per member is that of the single-grid adjoint, so a member's
-// gradient is bit-identical to chimera_adj_segment_vjp's.
+// gradient is bit-identical to chimera_adj_segment_vjp's.  The recompute also keeps
+// each stage's Up (the state after the flux update), which the reverse reads instead
+// of recomputing the tile's fluxes.  chimera_badj_sweep runs the segments of a whole
+// run from last to first in one call.
 #include <algorithm>
 #include <numeric>
 #include <vector>
@@ -23,7 +26,8 @@ namespace {
 struct BAdjMember {
     AdjGrid G;
     size_t n;
-    double *traj;               // 3 x cap slots x 6 fields x n
+    double *traj;               // (3 cap + 1) slots x 6 fields x n
+    double *uptraj;             // 3 cap slots x 4 fields x n: Up of step s, stage st at 3s+st-1
     const double *state0[6];    // the caller's U^0 of the segment (device)
     double *lam_io[6];          // the caller's lam (device, in / out)
     double *state_out[6];       // forward: where the last step's output goes
@@ -48,6 +52,10 @@ __device__ __forceinline__ void badj_segment(int ntiles, int &begin, int &end, i
 
 __device__ __forceinline__ double *slot_field(const BAdjMember &m, int slot, int q) {
     return m.traj + ((size_t)6 * slot + q) * m.n;
+}
+
+__device__ __forceinline__ double *slot_up(const BAdjMember &m, int slot, int q) {
+    return m.uptraj + ((size_t)4 * slot + q) * m.n;
 }
 
 // stage input of step s, stage st (1..3): U^s, U^(1), U^(2)
@@ -76,15 +84,15 @@ __device__ __forceinline__ GridView member_view(const BAdjMember &m, int s, int 
     return g;
 }
 
-// Forward stage st of step s for every member.  Segment recompute (pure = 0): the
-// stage output goes to the trajectory, which keeps every stage input.  Plain forward
-// (pure = 1): the running state is the member's state_out (U^0 read from state0), the
-// two intermediate stage states use trajectory slots 1 and 2, and stage 3 overwrites
-// state_out in place (it reads u^n only at the cell it writes).
-template <int SOLVER, int LIMITER, int TX, int TY>
+// Forward stage st of step s for every member.  Segment recompute (PURE false): the
+// stage output goes to the trajectory, which keeps every stage input, and Up to the Up
+// trajectory.  Plain forward (PURE): the running state is the member's state_out (U^0
+// read from state0), the two intermediate stage states use trajectory slots 1 and 2,
+// and stage 3 overwrites state_out in place (it reads u^n only at the cell it writes).
+template <int SOLVER, int LIMITER, int TX, int TY, bool PURE>
 __global__ __launch_bounds__(STAGE_THREADS) void badj_forward_kernel(
     const BAdjMember *__restrict__ mem, const BTile *__restrict__ tiles, int ntiles, int s, int st,
-    int pure, int use_lim, int mood) {
+    int use_lim, int mood) {
     __shared__ double lds[TileShape<TX, TY>::LDS_DOUBLES];
     int begin, end, slot, slots;
     badj_segment(ntiles, begin, end, slot, slots);
@@ -94,7 +102,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void badj_forward_kernel(
         GridView g = member_view(m, s, st);
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            if (pure) {
+            if (PURE) {
                 const double *un = s == 0 ? m.state0[q] : m.state_out[q];
                 g.uin[q] = st == 1 ? un : slot_field(m, st - 1, q);
                 g.u0[q] = un;
@@ -103,7 +111,15 @@ __global__ __launch_bounds__(STAGE_THREADS) void badj_forward_kernel(
                 g.uout[q] = slot_field(m, 3 * s + st, q);
             }
         }
-        stage_tile<SOLVER, LIMITER, TX, TY>(g, st, use_lim != 0, mood != 0, tr.tx, tr.ty, lds);
+        if (PURE) {
+            stage_tile<SOLVER, LIMITER, TX, TY>(g, st, use_lim != 0, mood != 0, tr.tx, tr.ty, lds);
+        } else {
+            StageStore store{};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) store.Up[q] = slot_up(m, 3 * s + st - 1, q);
+            stage_tile<SOLVER, LIMITER, TX, TY, STORE_UP>(g, st, use_lim != 0, mood != 0, tr.tx,
+                                                          tr.ty, lds, &store);
+        }
         __syncthreads();   // the next tile reuses LDS
     }
 }
@@ -129,13 +145,16 @@ __global__ __launch_bounds__(TX *TY) void badj_rev_a_kernel(const BAdjMember *__
         const BTile tr = tiles[t];
         const BAdjMember &m = mem[tr.member];
         const GridView g = member_view(m, s, st);
-        const double *lout[6];
+        const double *lout[6], *up[4];
         double *lin[6], *lu0[6];
         stage_lams(m, st, lout, lin);
 #pragma unroll
         for (int q = 0; q < 6; ++q) lu0[q] = m.lam[2][q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) up[q] = slot_up(m, 3 * s + st - 1, q);
         double dt = m.dts[s];
-        rev_tile_a<SOLVER, LIMITER, TX, TY>(g, m.G, m.F, lout, lin, lu0, st, dt, tr.tx, tr.ty, lds);
+        rev_tile_a<SOLVER, LIMITER, TX, TY>(g, m.G, m.F, up, lout, lin, lu0, st, dt, tr.tx, tr.ty,
+                                            lds);
         __syncthreads();
     }
 }
@@ -168,19 +187,20 @@ constexpr int FTX = 16, FTY = 16;
 constexpr int BADJ_NUM_SHAPES = 2;
 constexpr int BADJ_SHAPES[BADJ_NUM_SHAPES][2] = {{16, 16}, {32, 8}};
 
-using BFwdFn = void (*)(const BAdjMember *, const BTile *, int, int, int, int, int, int);
+using BFwdFn = void (*)(const BAdjMember *, const BTile *, int, int, int, int, int);
 using BRevAFn = void (*)(const BAdjMember *, const BTile *, int, int, int);
 using BRevBFn = void (*)(const BAdjMember *, const BTile *, int, int, int);
 
-template <int S>
+template <int S, bool PURE>
 constexpr BFwdFn fwd_of(int limiter) {
-    return limiter == LIM_MC ? badj_forward_kernel<S, LIM_MC, FTX, FTY>
-                             : badj_forward_kernel<S, LIM_VAN_LEER, FTX, FTY>;
+    return limiter == LIM_MC ? badj_forward_kernel<S, LIM_MC, FTX, FTY, PURE>
+                             : badj_forward_kernel<S, LIM_VAN_LEER, FTX, FTY, PURE>;
 }
+template <bool PURE>
 BFwdFn fwd_fn(int solver, int limiter) {
-    return solver == RUSANOV ? fwd_of<RUSANOV>(limiter)
-         : solver == HLLE    ? fwd_of<HLLE>(limiter)
-                             : fwd_of<HLLD>(limiter);
+    return solver == RUSANOV ? fwd_of<RUSANOV, PURE>(limiter)
+         : solver == HLLE    ? fwd_of<HLLE, PURE>(limiter)
+                             : fwd_of<HLLD, PURE>(limiter);
 }
 
 template <int S, int TX, int TY>
@@ -219,6 +239,7 @@ struct chimera_badj {
     BAdjMember *dmem = nullptr;
     double *arena = nullptr;       // per member: lam buffers and exchange arrays
     std::vector<double *> traj;    // per member trajectory (grown on demand)
+    std::vector<double *> uptraj;  // per member Up trajectory
     int traj_cap = 0;              // steps the trajectories hold
     double *dts = nullptr;         // members x dt_cap
     int dt_cap = 0;
@@ -227,6 +248,9 @@ struct chimera_badj {
     int shape_a = 1, shape_b = 1;
     int cu_count = 0;
     long long cells = 0;
+    // the whole-run sweep: one member table per segment
+    BAdjMember *dseg = nullptr;
+    int dseg_cap = 0;              // segments the tables hold
 };
 
 namespace {
@@ -272,6 +296,7 @@ int badj_upload(chimera_badj *b, const double *const *state0, double *const *lam
             h.state_out[q] = state_out ? state_out[6 * m + q] : nullptr;
         }
         h.traj = b->traj[m];
+        h.uptraj = b->uptraj[m];
         h.dts = b->dts + (size_t)m * b->dt_cap;
     }
     CH_HIP(hipStreamSynchronize(b->stream));
@@ -295,15 +320,20 @@ int badj_prepare(chimera_badj *b, const double *dt_in, int nsteps, int traj_step
         for (int s = 0; s < nsteps; ++s) h[(size_t)m * b->dt_cap + s] = dt_in[(size_t)m * nsteps + s];
     CH_HIP(hipMemcpy(b->dts, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     if (traj_steps > b->traj_cap) {
-        for (auto &p : b->traj) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
+        for (auto *v : {&b->traj, &b->uptraj})
+            for (auto &p : *v) {
+                (void)hipFree(p);
+                p = nullptr;
+            }
         b->traj_cap = 0;
         for (int m = 0; m < b->members; ++m) {
-            const size_t bytes = (size_t)3 * traj_steps * 6 * b->hmem[m].n * sizeof(double);
-            if (hipMalloc(&b->traj[m], bytes) != hipSuccess) {
-                set_error("batched adjoint: trajectory of member %d (%zu bytes) failed", m, bytes);
+            const size_t n = b->hmem[m].n;
+            const size_t bytes = (size_t)(3 * traj_steps + 1) * 6 * n * sizeof(double);
+            const size_t ubytes = (size_t)3 * traj_steps * 4 * n * sizeof(double);
+            if (hipMalloc(&b->traj[m], bytes) != hipSuccess ||
+                hipMalloc(&b->uptraj[m], ubytes) != hipSuccess) {
+                set_error("batched adjoint: trajectory of member %d (%zu bytes) failed", m,
+                          bytes + ubytes);
                 return CHIMERA_ENOMEM;
             }
         }
@@ -312,22 +342,52 @@ int badj_prepare(chimera_badj *b, const double *dt_in, int nsteps, int traj_step
     return CHIMERA_OK;
 }
 
-// nsteps forward steps of every member from state0.  pure: into state_out, keeping
-// nothing; else the segment recompute, keeping every stage input in the trajectories
-// (the last step's third stage, whose output the reverse does not need, is skipped).
-int badj_forward(chimera_badj *b, int nsteps, bool pure) {
-    const BFwdFn fn = fwd_fn(b->sch.riemann_solver, b->sch.slope_limiter);
+// nsteps forward steps of every member of the table mem from its state0.  pure: into
+// state_out, keeping nothing; else the segment recompute, keeping every stage input
+// and every stage's Up in the trajectories.
+int badj_forward(chimera_badj *b, const BAdjMember *mem, int nsteps, bool pure) {
+    const BFwdFn fn = pure ? fwd_fn<true>(b->sch.riemann_solver, b->sch.slope_limiter)
+                           : fwd_fn<false>(b->sch.riemann_solver, b->sch.slope_limiter);
     int grid;
     CH_TRY(badj_grid(b, (const void *)fn, STAGE_THREADS, grid));
-    for (int s = 0; s < nsteps; ++s) {
-        const bool last = s + 1 == nsteps;
-        for (int st = 1; st <= 3; ++st) {
-            if (st == 3 && last && !pure) break;
-            hipLaunchKernelGGL(fn, dim3(grid), dim3(STAGE_THREADS), 0, b->stream, b->dmem,
-                               b->dtiles[0], b->ntiles[0], s, st, pure ? 1 : 0,
-                               b->sch.use_slope_limiting, b->sch.fallback_2_mood);
+    for (int s = 0; s < nsteps; ++s)
+        for (int stage = 1; stage <= 3; ++stage) {
+            hipLaunchKernelGGL(fn, dim3(grid), dim3(STAGE_THREADS), 0, b->stream, mem,
+                               b->dtiles[0], b->ntiles[0], s, stage, b->sch.use_slope_limiting,
+                               b->sch.fallback_2_mood);
             CH_TRY(launch_check("badj_forward_kernel"));
         }
+    return CHIMERA_OK;
+}
+
+// The reverse of a recomputed segment of nsteps steps for every member of the table mem,
+// on the main stream: per step, newest first, the fused pair for stages 3, 2, 1.
+int badj_reverse(chimera_badj *b, const BAdjMember *mem, int nsteps) {
+    const BRevAFn fa = reva_fn(b->sch.riemann_solver, b->sch.slope_limiter, b->shape_a);
+    const BRevBFn fb = revb_fn(b->sch.slope_limiter, b->shape_b);
+    const int ta = BADJ_SHAPES[b->shape_a][0] * BADJ_SHAPES[b->shape_a][1];
+    const int tb = BADJ_SHAPES[b->shape_b][0] * BADJ_SHAPES[b->shape_b][1];
+    int ga, gb;
+    CH_TRY(badj_grid(b, (const void *)fa, ta, ga));
+    CH_TRY(badj_grid(b, (const void *)fb, tb, gb));
+    for (int s = nsteps - 1; s >= 0; --s)
+        for (int st = 3; st >= 1; --st) {
+            hipLaunchKernelGGL(fa, dim3(ga), dim3(ta), 0, b->stream, mem,
+                               b->dtiles[1 + b->shape_a], b->ntiles[1 + b->shape_a], s, st);
+            hipLaunchKernelGGL(fb, dim3(gb), dim3(tb), 0, b->stream, mem,
+                               b->dtiles[1 + b->shape_b], b->ntiles[1 + b->shape_b], s, st);
+            CH_TRY(launch_check("batched adjoint reverse stage"));
+        }
+    return CHIMERA_OK;
+}
+
+int ensure_sweep(chimera_badj *b, int nseg) {
+    if (nseg > b->dseg_cap) {
+        (void)hipFree(b->dseg);
+        b->dseg = nullptr;
+        b->dseg_cap = 0;
+        CH_HIP(hipMalloc(&b->dseg, (size_t)nseg * b->members * sizeof(BAdjMember)));
+        b->dseg_cap = nseg;
     }
     return CHIMERA_OK;
 }
@@ -338,9 +398,11 @@ extern "C" int chimera_badj_destroy(chimera_badj_t *b) {
     if (!b) return CHIMERA_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (auto p : b->traj) (void)hipFree(p);
+    for (auto p : b->uptraj) (void)hipFree(p);
     for (auto p : b->dtiles) (void)hipFree(p);
     (void)hipFree(b->arena);
     (void)hipFree(b->dmem);
+    (void)hipFree(b->dseg);
     (void)hipFree(b->dts);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -358,6 +420,7 @@ extern "C" int chimera_badj_create(chimera_badj_t **out, int members, const int 
     b->sch = *scheme;
     b->hmem.resize(members);
     b->traj.assign(members, nullptr);
+    b->uptraj.assign(members, nullptr);
     size_t total = 0;
     for (int m = 0; m < members; ++m) {
         if (!(nx[m] >= 4 && ny[m] >= 4 && dx[m] > 0.0 && gamma[m] > 1.0)) {
@@ -428,7 +491,7 @@ extern "C" int chimera_badj_forward(chimera_badj_t *b, const double *const *stat
         CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
     CH_TRY(badj_prepare(b, dt_in, nsteps, 1));   // slots 1 and 2 of one step
     CH_TRY(badj_upload(b, state_in, nullptr, state_out));
-    CH_TRY(badj_forward(b, nsteps, true));
+    CH_TRY(badj_forward(b, b->dmem, nsteps, true));
     CH_HIP(hipStreamSynchronize(b->stream));
     return CHIMERA_OK;
 }
@@ -444,22 +507,55 @@ extern "C" int chimera_badj_segment_vjp(chimera_badj_t *b, const double *const *
         CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
     CH_TRY(badj_prepare(b, dt_in, nsteps, nsteps));
     CH_TRY(badj_upload(b, state0, lam, nullptr));
-    CH_TRY(badj_forward(b, nsteps, false));
-    const BRevAFn fa = reva_fn(b->sch.riemann_solver, b->sch.slope_limiter, b->shape_a);
-    const BRevBFn fb = revb_fn(b->sch.slope_limiter, b->shape_b);
-    const int ta = BADJ_SHAPES[b->shape_a][0] * BADJ_SHAPES[b->shape_a][1];
-    const int tb = BADJ_SHAPES[b->shape_b][0] * BADJ_SHAPES[b->shape_b][1];
-    int ga, gb;
-    CH_TRY(badj_grid(b, (const void *)fa, ta, ga));
-    CH_TRY(badj_grid(b, (const void *)fb, tb, gb));
-    for (int s = nsteps - 1; s >= 0; --s)
-        for (int st = 3; st >= 1; --st) {
-            hipLaunchKernelGGL(fa, dim3(ga), dim3(ta), 0, b->stream, b->dmem,
-                               b->dtiles[1 + b->shape_a], b->ntiles[1 + b->shape_a], s, st);
-            hipLaunchKernelGGL(fb, dim3(gb), dim3(tb), 0, b->stream, b->dmem,
-                               b->dtiles[1 + b->shape_b], b->ntiles[1 + b->shape_b], s, st);
-            CH_TRY(launch_check("batched adjoint reverse stage"));
+    CH_TRY(badj_forward(b, b->dmem, nsteps, false));
+    CH_TRY(badj_reverse(b, b->dmem, nsteps));
+    CH_HIP(hipStreamSynchronize(b->stream));
+    return CHIMERA_OK;
+}
+
+// The whole reverse sweep of an nsteps run checkpointed every seg_steps steps for every
+// member: states holds nseg x members x 6 device pointers (checkpoint k of member m at
+// 6 (k members + m)), dt_in members x nsteps (host), lam members x 6 device pointers
+// (in place).  The segments are recomputed and reversed from last to first in one
+// call, with one member table per segment.  (Overlapping a recompute with the
+// previous segment's reverse on a second stream gains nothing here: an ensemble's
+// reverse passes already hold every CU, and two segments' trajectories of a
+// 64-member ensemble would not fit in HBM.)
+extern "C" int chimera_badj_sweep(chimera_badj_t *b, const double *const *states,
+                                  const double *dt_in, int nsteps, int seg_steps,
+                                  double *const *lam) {
+    CH_REQUIRE(b && states && dt_in && lam, "NULL argument");
+    CH_REQUIRE(nsteps >= 1 && seg_steps >= 1, "nsteps and seg_steps must be >= 1");
+    for (long long k = 0; k < (long long)b->members * nsteps; ++k)
+        CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
+    const int nseg = (nsteps + seg_steps - 1) / seg_steps;
+    for (long long k = 0; k < 6LL * nseg * b->members; ++k)
+        CH_REQUIRE(states[k], "states[%lld] is NULL", k);
+    CH_TRY(badj_prepare(b, dt_in, nsteps, std::min(seg_steps, nsteps)));
+    CH_TRY(ensure_sweep(b, nseg));
+    // one member table per segment: its checkpoint and dt offset
+    std::vector<BAdjMember> tabs((size_t)nseg * b->members);
+    for (int k = 0; k < nseg; ++k)
+        for (int m = 0; m < b->members; ++m) {
+            BAdjMember h = b->hmem[m];
+            for (int q = 0; q < 6; ++q) {
+                h.state0[q] = states[6 * ((size_t)k * b->members + m) + q];
+                h.lam_io[q] = lam[6 * (size_t)m + q];
+                h.state_out[q] = nullptr;
+            }
+            h.traj = b->traj[m];
+            h.uptraj = b->uptraj[m];
+            h.dts = b->dts + (size_t)m * b->dt_cap + (size_t)k * seg_steps;
+            tabs[(size_t)k * b->members + m] = h;
         }
+    CH_HIP(hipStreamSynchronize(b->stream));
+    CH_HIP(hipMemcpy(b->dseg, tabs.data(), tabs.size() * sizeof(BAdjMember), hipMemcpyHostToDevice));
+    for (int k = nseg - 1; k >= 0; --k) {
+        const BAdjMember *tab = b->dseg + (size_t)k * b->members;
+        const int len = std::min(seg_steps, nsteps - k * seg_steps);
+        CH_TRY(badj_forward(b, tab, len, false));
+        CH_TRY(badj_reverse(b, tab, len));
+    }
     CH_HIP(hipStreamSynchronize(b->stream));
     return CHIMERA_OK;
 }

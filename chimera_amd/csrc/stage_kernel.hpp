@@ -39,8 +39,9 @@ struct GridView {
     double dx, vol, gamma;
 };
 
-// Stage intermediates the discrete adjoint reads back (owned cells only; written
-// only by the STORE instantiation of stage_tile).
+// Stage intermediates the discrete adjoint reads back (owned cells only): all of them
+// with STORE_ALL, only Up with STORE_UP (the segment recompute of the fused reverse).
+enum StoreMode { STORE_NONE = 0, STORE_ALL = 1, STORE_UP = 2 };
 struct StageStore {
     double *P[6];        // stage primitives rho, vx, vy, P, Bx, By
     double *face[24];    // field f, side s (XL, XR, YL, YR) at 4*f + s, after the thermal check
@@ -192,7 +193,7 @@ __device__ __forceinline__ bool thermal_reset(double f[6], const double c[6]) {
     return false;
 }
 
-template <int SOLVER, int LIMITER, int TX, int TY, bool STORE = false>
+template <int SOLVER, int LIMITER, int TX, int TY, int STORE = STORE_NONE>
 __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool use_lim, bool mood,
                                            int tile_x, int tile_y, double *lds,
                                            const StageStore *store = nullptr) {
@@ -228,7 +229,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         sP[3 * S::P_SZ + idx] = P;
         sP[4 * S::P_SZ + idx] = Bx;
         sP[5 * S::P_SZ + idx] = By;
-        if (STORE) {
+        if (STORE == STORE_ALL) {
             int ci = lx - 2, cj = ly - 2;
             if (ci >= 0 && ci < TX && cj >= 0 && cj < TY && x0 + ci < nx && y0 + cj < ny) {
                 size_t ko = (size_t)(x0 + ci) + (size_t)nx * (y0 + cj);
@@ -296,7 +297,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             if (thermal_reset(R, C)) bits |= 1u << (25 + 2 * dir);
             const int i = fx - 1, j = fy - 1;
             const bool owned = i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny;
-            if (STORE && owned) {
+            if (STORE == STORE_ALL && owned) {
                 size_t ko = (size_t)(x0 + i) + (size_t)nx * (y0 + j);
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
@@ -350,7 +351,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             if (ylo_open) v = v - dtdx * ft;
             else v = v - dtdx * ft + dtdx * sFY[q * S::FY_SZ + yr - S::FYW];
             U[q] = v;
-            if (STORE) store->Up[q][k] = v;
+            if (STORE != STORE_NONE) store->Up[q][k] = v;
         }
         const int pc = (cj + 2) * S::PW + (ci + 2);
         const double Bxc = sP[4 * S::P_SZ + pc], Byc = sP[5 * S::P_SZ + pc];
@@ -395,7 +396,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             }
         }
 
-        if (STORE) {
+        if (STORE == STORE_ALL) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) store->U3[q][k] = U[q];
         }

@@ -265,7 +265,8 @@ int chimera_adj_step_vjp(chimera_adj_t *adj, const double *const *state_n, doubl
                          const double *const *lam_out, double *const *lam_in);
 /* Reverse sweep over one checkpoint segment of nsteps steps: recompute the segment from
  * state0 = U^0 with the fused forward stage kernel and the dts dt_in[0..nsteps-1] (host
- * memory), keeping every stage input (3 * nsteps states of device memory, allocated on
+ * memory), keeping every stage input and every stage's flux-updated conserved state
+ * (3 * nsteps + 1 states and 3 * nsteps four-field sets of device memory, allocated on
  * first use and kept), then lam <- (dU^{nsteps}/dU^0)^T lam in place (device arrays). */
 int chimera_adj_segment_vjp(chimera_adj_t *adj, const double *const *state0, const double *dt_in,
                             int nsteps, double *const *lam);
@@ -273,9 +274,9 @@ int chimera_adj_segment_vjp(chimera_adj_t *adj, const double *const *state0, con
  * holds 6 device pointers per checkpoint (checkpoint k = U^{k * seg_steps}, k = 0 ..
  * ceil(nsteps / seg_steps) - 1), then lam <- (dU^{nsteps}/dU^0)^T lam in place.  The
  * same bits as chimera_adj_segment_vjp over the segments from last to first; the
- * recompute of segment k-1 runs on a second stream while segment k is reversed (two
- * segments of 3 * seg_steps states of device memory).  Replaces the per-segment loop
- * of the autograd backward; no counterpart in the reference (it has no derivatives). */
+ * recompute of segment k-1 runs on a second stream while segment k is reversed (twice
+ * the trajectory memory of one segment).  Replaces the per-segment loop of the
+ * autograd backward; no counterpart in the reference (it has no derivatives). */
 int chimera_adj_sweep(chimera_adj_t *adj, const double *const *states, const double *dt_in,
                       int nsteps, int seg_steps, double *const *lam);
 void *chimera_adj_stream(chimera_adj_t *adj);
@@ -316,10 +317,19 @@ int chimera_badj_destroy(chimera_badj_t *badj);
 int chimera_badj_forward(chimera_badj_t *badj, const double *const *state_in, const double *dt_in,
                          int nsteps, double *const *state_out);
 /* One checkpoint segment of every member: recompute from state0 keeping every stage
- * input (3 * nsteps states per member, allocated on first use and kept), then
+ * input and every stage's flux-updated conserved state (per member 3 * nsteps + 1
+ * states and 3 * nsteps four-field sets, allocated on first use and kept), then
  * lam <- (dU^{nsteps}/dU^0)^T lam in place. */
 int chimera_badj_segment_vjp(chimera_badj_t *badj, const double *const *state0,
                              const double *dt_in, int nsteps, double *const *lam);
+/* The whole reverse sweep of an nsteps run checkpointed every seg_steps steps, for every
+ * member: states holds nseg x members x 6 device pointers (checkpoint k of member m at
+ * 6 * (k * members + m), k = 0 .. ceil(nsteps / seg_steps) - 1), dt_in members x nsteps
+ * (host), lam members x 6 device pointers, updated in place.  The bits of
+ * chimera_badj_segment_vjp over the segments from last to first, in one call.  No
+ * counterpart in the reference (it has no derivatives). */
+int chimera_badj_sweep(chimera_badj_t *badj, const double *const *states, const double *dt_in,
+                       int nsteps, int seg_steps, double *const *lam);
 /* Tile shapes of the two reverse passes: 0 = 16x16, 1 = 32x8 (default; -1 keeps). */
 int chimera_badj_set_tiles(chimera_badj_t *badj, int shape_a, int shape_b);
 void *chimera_badj_stream(chimera_badj_t *badj);

@@ -405,3 +405,32 @@ def test_sweep_matches_segment_loop(gpu_lib, scheme, nsteps, K):
         assert np.array_equal(got.cpu().numpy(), ref.cpu().numpy())
     adj.close()
     assert np.all(np.isfinite(ref.cpu().numpy()))
+
+
+@pytest.mark.parametrize("nsteps,K", [(11, 4), (5, 8)])
+def test_batched_sweep_matches_segment_loop(gpu_lib, nsteps, K):
+    """The pipelined batched sweep against chimera_badj_segment_vjp over the segments
+    from last to first, bit for bit, twice on one workspace."""
+    torch = _torch()
+    from chimera_amd.adjoint import BatchAdjoint
+    scheme = (3, 2, 1, 1)
+    probs = _batch_members(scheme)
+    dts = _member_dts(probs, nsteps)
+    badj = BatchAdjoint([(p.nx, p.ny, p.dx, p.gamma, p.bc) for p in probs], scheme)
+    states = [torch.from_numpy(p.state.copy()).cuda() for p in probs]
+    ckpts = []
+    for s0 in range(0, nsteps, K):
+        ckpts.append([x.clone() for x in states])
+        states = badj.forward(states, dts[:, s0:s0 + K])
+    rng = np.random.default_rng(13)
+    lam0 = [torch.from_numpy(rng.standard_normal(p.state.shape)).cuda() for p in probs]
+    ref = [l.clone() for l in lam0]
+    for k in reversed(range(len(ckpts))):
+        badj.segment_vjp(ckpts[k], np.ascontiguousarray(dts[:, k * K:(k + 1) * K]), ref)
+    print(f"batched sweep: {nsteps} steps, {len(ckpts)} segments", flush=True)
+    for _ in range(2):
+        got = [l.clone() for l in lam0]
+        badj.sweep(ckpts, dts, K, got)
+        for m in range(len(probs)):
+            assert np.array_equal(got[m].cpu().numpy(), ref[m].cpu().numpy()), m
+    badj.close()
