@@ -216,6 +216,7 @@ def adjoint_field_loop(args, torch) -> dict:
     g.sync()
     t_fwd = time.perf_counter() - t0
     g.close()
+    gradient(prob, Az, dts[:K], T)   # warm-up: the backward's workspace for one segment
     grad, loss_v, t_evolve, t_bwd = gradient(prob, Az, dts, T)
     grad_finite = bool(np.isfinite(grad).all())
 
@@ -326,6 +327,11 @@ def differentiable_grf(args, torch) -> dict:
         badj.forward(st0s, dts)
         torch.cuda.synchronize()
         t_fwd = time.perf_counter() - t0
+        # warm-up of the backward: one checkpoint segment allocates the trajectories
+        wk = min(50, dts.shape[1])
+        U0w = [u.detach().clone().requires_grad_(True) for u in st0s]
+        sum(o.sum() for o in evolve_batch(U0w, np.ascontiguousarray(dts[:, :wk]), badj,
+                                          checkpoint_every=50)).backward()
         ws = [w.clone().requires_grad_(True) for w in noise]
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -111,6 +111,28 @@ def record_dt(prob, nsteps: int, dt_mode: int = DT_CFL) -> np.ndarray:
     return dts
 
 
+# adjoint workspaces of the autograd backward, one per (device, geometry): the
+# trajectory memory of a checkpoint segment is allocated on the first backward and
+# reused by the next ones
+_WORKSPACES: dict = {}
+
+
+def _workspace(geom) -> AdjointStepper:
+    key = (torch.cuda.current_device(), geom)
+    adj = _WORKSPACES.get(key)
+    if adj is None:
+        nx, ny, dx, gamma, bc, scheme = geom
+        adj = _WORKSPACES[key] = AdjointStepper(nx, ny, dx, gamma, bc, scheme)
+    return adj
+
+
+def release_workspaces() -> None:
+    """Free the device memory the autograd backward of `evolve` keeps between calls."""
+    for adj in _WORKSPACES.values():
+        adj.close()
+    _WORKSPACES.clear()
+
+
 class _Evolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U0, dts, geom, checkpoint_every):
@@ -134,14 +156,8 @@ class _Evolve(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gN):
         ckpts = ctx.saved_tensors
-        dts, K = ctx.dts, ctx.K
-        nx, ny, dx, gamma, bc, scheme = ctx.geom
-        adj = AdjointStepper(nx, ny, dx, gamma, bc, scheme)
         lam = gN.contiguous().clone()
-        try:
-            adj.sweep(ckpts, dts, K, lam)
-        finally:
-            adj.close()
+        _workspace(ctx.geom).sweep(ckpts, ctx.dts, ctx.K, lam)
         return lam, None, None, None
 
 
