@@ -35,17 +35,17 @@ __device__ __forceinline__ void cons_to_prim_vjp(double M, double Mx, double My,
     double lke = lq * E_FLOOR_FRAC;
     double lp0 = p1_is_p0 ? lp1 : 0.0;
     double gm1 = g - 1.0;
-    lU[3] += lp0 * gm1 / vol;
+    const double ivol = 1.0 / vol, ir = 1.0 / r, irv = ir * ivol;   // reverse only
+    lU[3] += lp0 * gm1 * ivol;
     lke += -lp0 * gm1;
     lhalf += lp0 * (1.0 - gm1);
     lr += lke * 0.5 * (u * u + w * w);
     lu += lke * r * u;
     lw += lke * r * w;
-    double rv = r * vol;
-    lU[1] += lu / rv;
-    lU[2] += lw / rv;
-    lr += -(lu * u + lw * w) / r;
-    if (r_is_a) lU[0] += lr / vol;
+    lU[1] += lu * irv;
+    lU[2] += lw * irv;
+    lr += -(lu * u + lw * w) * ir;
+    if (r_is_a) lU[0] += lr * ivol;
     lBx += lhalf * Bx;
     lBy += lhalf * By;
 }

@@ -33,7 +33,7 @@ __device__ __forceinline__ SideRev side_energy(const FaceState &s, double g) {
 __device__ __forceinline__ void side_energy_rev(const FaceState &s, const SideRev &r, double g,
                                                 double len, double lpth, double lhalf,
                                                 double l[6]) {
-    lpth += len / (g - 1.0);
+    lpth += len * (1.0 / (g - 1.0));
     l[0] += len * (0.5 * (s.u * s.u + s.w * s.w));
     l[1] += len * (s.rho * s.u);
     l[2] += len * (s.rho * s.w);
@@ -179,17 +179,18 @@ __device__ __forceinline__ void hlle_vjp(const FaceState &L, const FaceState &R,
         lcfL += lSR;
     }
     // cf = fmin(sqrt(0.5 (x + |x|)), cap); x = (g pth + 2 halfB2) / rho
+    // (lam of x) / rho with one division per side
     if (!capL && xL > 0.0 && sqL > 0.0) {
-        double lx = lcfL * (0.5 / sqL);
-        lpthL += lx * g / L.rho;
-        lhL += lx * 2.0 / L.rho;
-        lamL[0] -= lx * xL / L.rho;
+        const double lxr = lcfL * (0.5 / (sqL * L.rho));
+        lpthL += lxr * g;
+        lhL += lxr * 2.0;
+        lamL[0] -= lxr * xL;
     }
     if (!capR && xR > 0.0 && sqR > 0.0) {
-        double lx = lcfR * (0.5 / sqR);
-        lpthR += lx * g / R.rho;
-        lhR += lx * 2.0 / R.rho;
-        lamR[0] -= lx * xR / R.rho;
+        const double lxr = lcfR * (0.5 / (sqR * R.rho));
+        lpthR += lxr * g;
+        lhR += lxr * 2.0;
+        lamR[0] -= lxr * xR;
     }
     side_energy_rev(L, eL, g, lenL, lpthL, lhL, lamL);
     side_energy_rev(R, eR, g, lenR, lpthR, lhR, lamR);
@@ -285,17 +286,18 @@ __device__ __forceinline__ void rusanov_vjp(const FaceState &L, const FaceState 
     double lpthL = 0.0, lpthR = 0.0, lhL = 0.0, lhR = 0.0;
     if (L.u != 0.0) lamL[1] += lsL * copysign(1.0, L.u);
     if (R.u != 0.0) lamR[1] += lsR * copysign(1.0, R.u);
+    // (lam of c2) / rho with one division per side
     if (cL2 > 0.0 && sqL > 0.0) {
-        double lc = lsL * (0.5 / sqL);
-        lpthL += lc * g / L.rho;
-        lhL += lc * 2.0 / L.rho;
-        lamL[0] -= lc * cL2 / L.rho;
+        const double lcr = lsL * (0.5 / (sqL * L.rho));
+        lpthL += lcr * g;
+        lhL += lcr * 2.0;
+        lamL[0] -= lcr * cL2;
     }
     if (cR2 > 0.0 && sqR > 0.0) {
-        double lc = lsR * (0.5 / sqR);
-        lpthR += lc * g / R.rho;
-        lhR += lc * 2.0 / R.rho;
-        lamR[0] -= lc * cR2 / R.rho;
+        const double lcr = lsR * (0.5 / (sqR * R.rho));
+        lpthR += lcr * g;
+        lhR += lcr * 2.0;
+        lamR[0] -= lcr * cR2;
     }
     // averages
     double lra = -linv * inv * inv;
@@ -396,13 +398,14 @@ __device__ __forceinline__ void hlld_cf_rev(const HlldSide &h, double g, double 
     const double lca = -ld * (4.0 * h.cs2);
     lcs2 += ltmp;
     const double lb2 = ltmp;
-    a.lB2 += lb2 / h.r;
-    a.l[5] += lb2 * (2.0 * h.C) / h.r;
-    a.l[0] -= lb2 * h.b2 / h.r;
-    a.lpth += lcs2 * g / h.r;
-    a.l[0] -= lcs2 * h.cs2 / h.r;
-    a.lB2 += lca / h.r;
-    a.l[0] -= lca * h.ca / h.r;
+    const double ir = 1.0 / h.r;   // one reciprocal for the seven quotients below
+    a.lB2 += lb2 * ir;
+    a.l[5] += lb2 * (2.0 * h.C) * ir;
+    a.l[0] -= lb2 * h.b2 * ir;
+    a.lpth += lcs2 * g * ir;
+    a.l[0] -= lcs2 * h.cs2 * ir;
+    a.lB2 += lca * ir;
+    a.l[0] -= lca * h.ca * ir;
 }
 
 // phys flux, v.B, B^2 and energy of one side -> its inputs (written into lam)
@@ -458,9 +461,10 @@ __device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar 
                                               double &la) {
     if (!st.sq_floor && st.sq > 0.0) lrs += lsq * (0.5 / st.sq);
     double ldS = 0.0;
+    const double idS = 1.0 / st.dS;   // reverse only: one reciprocal per denominator
     // ens = numE / dS
-    const double lnumE = lens / st.dS;
-    ldS -= lens * st.ens / st.dS;
+    const double lnumE = lens * idS;
+    ldS -= lens * st.ens * idS;
     a.len += lnumE * (S - h.u);
     lS += lnumE * h.e.en;
     a.l[1] -= lnumE * h.e.en;
@@ -479,9 +483,10 @@ __device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar 
     double lk = 0.0;
     if (st.cond) {
         double lden = 0.0;
+        const double iden = 1.0 / st.den;
         // Bs = C (r t^2 - B^2) / den
-        const double lnb = lBs / st.den;
-        lden -= lBs * st.Bs / st.den;
+        const double lnb = lBs * iden;
+        lden -= lBs * st.Bs * iden;
         a.l[5] += lnb * (h.r * (st.t * st.t) - h.B2);
         const double lin = lnb * h.C;
         a.l[0] += lin * (st.t * st.t);
@@ -491,8 +496,8 @@ __device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar 
         a.l[1] -= lt;
         // vs = w - nv / den, nv = B C (SM - u)
         a.l[2] += lvs;
-        const double lnv = -lvs / st.den;
-        lden += lvs * (st.nv / st.den) / st.den;
+        const double lnv = -lvs * iden;
+        lden += lvs * (st.nv * iden) * iden;
         a.l[4] += lnv * h.C * (SM - h.u);
         a.l[5] += lnv * h.B * (SM - h.u);
         lSM += lnv * (h.B * h.C);
@@ -507,8 +512,8 @@ __device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar 
     // k = a dS, rs = a / dS, dS = S - SM
     la += lk * st.dS;
     ldS += lk * st.a;
-    la += lrs / st.dS;
-    ldS -= lrs * st.rs / st.dS;
+    la += lrs * idS;
+    ldS -= lrs * st.rs * idS;
     lS += ldS;
     lSM -= ldS;
 }
@@ -726,8 +731,9 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
                     lvss += lvdotBss * Bss;
                     lBss += lvdotBss * vss;
                     double ldss = 0.0;
-                    const double lnB = lBss / dss;
-                    ldss -= lBss * Bss / dss;
+                    const double idss = 1.0 / dss;   // reverse only
+                    const double lnB = lBss * idss;
+                    ldss -= lBss * Bss * idss;
                     const double dv = sR.vs - sL.vs;
                     lsqL += lnB * sR.Bs;
                     lBsR += lnB * sL.sq;
@@ -737,8 +743,8 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
                     lsqR += lnB * sgn * sL.sq * dv;
                     lvsR += lnB * (sgn * sL.sq * sR.sq);
                     lvsL -= lnB * (sgn * sL.sq * sR.sq);
-                    const double lnv = lvss / dss;
-                    ldss -= lvss * vss / dss;
+                    const double lnv = lvss * idss;
+                    ldss -= lvss * vss * idss;
                     lsqL += lnv * sL.vs;
                     lvsL += lnv * sL.sq;
                     lsqR += lnv * sR.vs;
@@ -749,12 +755,13 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
                     lsqR += ldss;
                     // SLs = SM - |Bavg| / sqL, SRs = SM + |Bavg| / sqR
                     double labs = 0.0;
+                    const double isqL = 1.0 / sL.sq, isqR = 1.0 / sR.sq;
                     lSM += lSLs;
-                    labs -= lSLs / sL.sq;
-                    lsqL += lSLs * (absB / sL.sq) / sL.sq;
+                    labs -= lSLs * isqL;
+                    lsqL += lSLs * (absB * isqL) * isqL;
                     lSM += lSRs;
-                    labs += lSRs / sR.sq;
-                    lsqR -= lSRs * (absB / sR.sq) / sR.sq;
+                    labs += lSRs * isqR;
+                    lsqR -= lSRs * (absB * isqR) * isqR;
                     lBavg += labs * sgn;
                 } else {
                     lvsL += 0.5 * lvss;
