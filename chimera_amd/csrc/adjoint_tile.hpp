@@ -448,8 +448,11 @@ struct RevShapeB {
     static constexpr int CPT = (C_SZ + NT - 1) / NT;                 // cells [0, T] per thread
 };
 
-// lU0_add: the step's lam_u0, added to lam_in at the end (stage 1), else nullptr
-template <int LIMITER, int TX, int TY, int FPR = 1>
+// lU0_add: the step's lam_u0, added to lam_in at the end (stage 1), else nullptr.
+// EARLY_TAIL issues the tail's loads with the last round (a lone small grid, whose few
+// waves per SIMD cannot hide their latency); off, they are issued where they are used,
+// which keeps 56 VGPRs out of the field loop for grids with waves enough to hide it.
+template <int LIMITER, int TX, int TY, int FPR = 1, bool EARLY_TAIL = true>
 __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const *U,
                                            const FusedBufs &F, double *const *lin,
                                            const double *const *lU0_add, int tile_x, int tile_y,
@@ -527,8 +530,8 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
             }
         }
     };
-    // the tail's inputs, issued with the last round: per cell of [0, T] its B, lam of its
-    // cell B and its conserved U; for the owned cell of this thread lam_in (and lam_u0)
+    // the tail's inputs: per cell of [0, T] its B, lam of its cell B and its conserved U;
+    // for the owned cell of this thread lam_in (and lam_u0)
     size_t ck[B::CPT];
     bool cvalid[B::CPT];
     double tu[B::CPT][8];
@@ -536,21 +539,7 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
     const bool oact = x0 + ox < nx && y0 + oy < ny;
     const size_t ko = oact ? (size_t)(x0 + ox) + (size_t)nx * (y0 + oy) : 0;
     double tl[12];
-    auto load_tail = [&]() {
-#pragma unroll
-        for (int u = 0; u < B::CPT; ++u) {
-            const int c = tid + u * NT;
-            int i = 0, j = 0;
-            cvalid[u] = c < B::C_SZ && gcell(c % B::CW, c / B::CW, i, j);
-            ck[u] = (size_t)i + (size_t)nx * j;
-            if (!cvalid[u]) continue;
-            tu[u][0] = F.P(4)[ck[u]];
-            tu[u][1] = F.P(5)[ck[u]];
-            tu[u][2] = F.Bc(0)[ck[u]];
-            tu[u][3] = F.Bc(1)[ck[u]];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) tu[u][4 + q] = U[q][ck[u]];
-        }
+    auto load_owned = [&]() {
         if (oact) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) tl[q] = lin[q][ko];
@@ -559,6 +548,23 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
                 for (int q = 0; q < 6; ++q) tl[6 + q] = lU0_add[q][ko];
             }
         }
+    };
+    auto load_cell = [&](int u) {
+        const int c = tid + u * NT;
+        int i = 0, j = 0;
+        cvalid[u] = c < B::C_SZ && gcell(c % B::CW, c / B::CW, i, j);
+        ck[u] = (size_t)i + (size_t)nx * j;
+        if (!cvalid[u]) return;
+        tu[u][0] = F.P(4)[ck[u]];
+        tu[u][1] = F.P(5)[ck[u]];
+        tu[u][2] = F.Bc(0)[ck[u]];
+        tu[u][3] = F.Bc(1)[ck[u]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tu[u][4 + q] = U[q][ck[u]];
+    };
+    auto load_cells = [&]() {
+#pragma unroll
+        for (int u = 0; u < B::CPT; ++u) load_cell(u);
     };
     load(0, 0);
 
@@ -573,7 +579,10 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
                 if (tid + u * NT < B::P_SZ) sA[h * B::P_SZ + tid + u * NT] = pv[cur][h][u];
         __syncthreads();
         if (r + 1 < B::ROUNDS) load(r + 1, cur ^ 1);
-        else load_tail();
+        else if (EARLY_TAIL) {
+            load_cells();
+            load_owned();
+        }
         // slope parts of cells [-1, T+1] (as ar_slopes)
 #pragma unroll
         for (int h = 0; h < FPR; ++h) {
@@ -653,6 +662,7 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
     for (int u = 0; u < B::CPT; ++u) {
         const int c = tid + u * NT;
         if (c >= B::C_SZ) break;
+        if (!EARLY_TAIL) load_cell(u);
         const int lx = c % B::CW, ly = c / B::CW;
         if (!cvalid[u]) {
             sB[c] = 0.0;
@@ -671,6 +681,7 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
         sB[B::C_SZ + c] = lBy;
     }
     __syncthreads();
+    if (!EARLY_TAIL) load_owned();
     // face -> cell B average (as ar_bavg); lam_in of the owned cell written once:
     // ((lam_in + cell part) + lam_u0)
     if (oact) {
