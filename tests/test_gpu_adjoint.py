@@ -434,3 +434,35 @@ def test_batched_sweep_matches_segment_loop(gpu_lib, nsteps, K):
         for m in range(len(probs)):
             assert np.array_equal(got[m].cpu().numpy(), ref[m].cpu().numpy()), m
     badj.close()
+
+
+def test_evolve_workspace_reuse_is_stateless(gpu_lib):
+    """The autograd backward keeps its workspace between calls: a second backward, one
+    with another checkpoint interval (a different trajectory layout in the same
+    buffers) and one after release_workspaces() all give the first call's bits; the
+    interval itself does not change them."""
+    torch = _torch()
+    from chimera_amd.adjoint import cell_b2, evolve, field_loop_state, record_dt, release_workspaces
+    nx, ny, nsteps = 24, 20, 12
+    Az, dx, hyd = smooth_loop_parts(nx, ny)
+    p = problems.field_loop_from_az(Az, dx, scheme=(2, 2, 1, 1), hydro=hyd)
+    dts = record_dt(p, nsteps)
+    hyd_t = [torch.from_numpy(a).cuda() for a in hyd]
+
+    def grad(K):
+        Azg = torch.from_numpy(Az).cuda().requires_grad_(True)
+        UN = evolve(field_loop_state(Azg, dx, hydro=hyd_t), dts, p, checkpoint_every=K)
+        (cell_b2(UN) ** 2).sum().backward()
+        return Azg.grad.cpu().numpy()
+
+    release_workspaces()
+    g5 = grad(5)
+    assert np.all(np.isfinite(g5)) and np.any(g5)
+    assert np.array_equal(grad(5), g5)
+    g3 = grad(3)
+    assert np.array_equal(g3, g5)   # recomputation reproduces the forward states exactly
+    assert np.array_equal(grad(5), g5)
+    release_workspaces()
+    assert np.array_equal(grad(3), g3)
+    assert np.array_equal(grad(5), g5)
+    release_workspaces()
