@@ -3,21 +3,23 @@
 // the conservative update, constrained transport, the outflow correction, the conserved
 // floors and the RK blends.
 //
-// Per stage, the fused LDS-tiled stage kernel (stage_kernel.hpp, STORE instantiation)
-// recomputes the stage exactly as the forward path does and keeps what the reverse
-// needs -- primitives, face states after MOOD and the thermal check, branch bits, the
-// pre-floor and pre-blend states.  Reverse kernels ar_* are written in gather form
-// (each thread owns one output location and collects the contributions of every
-// forward operation that read it), so there are no atomics and the gradient is
-// deterministic.  Branches (limiter cases, MOOD resets, face and conserved floors,
-// thermal resets, Riemann-solver regions and catches) are the ones the forward pass
-// took.  The three Riemann solvers are reversed by hand (riemann_adj.hpp); their
-// forward-mode dual-number VJPs (riemann_dual.hpp) are the cross-check.
+// Two reverse paths give the same bits.  The default, fused one: the segment recompute
+// (stage_tile, STORE_UP) keeps every stage input and each stage's Up, and per stage two
+// LDS-tiled kernels (ar_tile_a / ar_tile_b over adjoint_tile.hpp) recompute the rest of
+// the stage tile by tile and reverse it.  The cross-check: the STORE_ALL instantiation
+// keeps every intermediate the reverse needs -- primitives, face states after MOOD and
+// the thermal check, branch bits, the pre-floor and pre-blend states -- and the
+// kernels ar_* read them back.  Both are in gather form (each thread owns one output
+// location and collects the contributions of every forward operation that read it), so
+// there are no atomics and the gradient is deterministic.  Branches (limiter cases,
+// MOOD resets, face and conserved floors, thermal resets, Riemann-solver regions and
+// catches) are the ones the forward pass took.  The three Riemann solvers are reversed
+// by hand (riemann_adj.hpp); their forward-mode dual-number VJPs (riemann_dual.hpp) are
+// the cross-check.
 //
 // A whole reverse sweep (chimera_adj_sweep) overlaps the recompute of checkpoint
 // segment k-1, on a second stream, with the reverse of segment k: the reverse kernels
-// leave most of the machine idle at sizes where one thread per cell gives a few waves
-// per SIMD.
+// leave SIMD slots idle at sizes where one thread per cell gives a few waves per SIMD.
 #include <algorithm>
 #include <vector>
 
@@ -420,7 +422,7 @@ struct chimera_adj {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     size_t n = 0;
-    int tiles_x = 0, ntiles = 0, grid = 0;   // STORE stage kernel launch shape
+    int tiles_x = 0, ntiles = 0, grid = 0;   // adjoint stage kernel launch shape
     double *arena = nullptr;
     StageBufs S[3];
     RevBufs R;
@@ -919,7 +921,7 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
     return CHIMERA_OK;
 }
 
-// One SSP-RK3 step with a given dt through the adjoint's forward (the STORE
+// One SSP-RK3 step with a given dt through the adjoint's forward (the STORE_ALL
 // instantiation of the fused stage kernel); device pointers.
 extern "C" int chimera_adj_forward_step(chimera_adj_t *a, const double *const *state_in,
                                         double dt, double *const *state_out) {

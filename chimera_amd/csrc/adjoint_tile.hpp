@@ -70,7 +70,7 @@ __device__ __forceinline__ void face_state_reverse(double lam[6], int s, bool to
 // reading stored intermediates (the stored-intermediate kernels ar_* of adjoint.hip are
 // the cross-check).
 //
-// rev_tile_a (one TX x TY tile of owned cells per workgroup) replaces the STORE
+// rev_tile_a (one TX x TY tile of owned cells per workgroup) replaces the STORE_ALL
 // recompute, ar_cell_tail, ar_corner and ar_faces: it loads the primitives of cells
 // [-2, TX+2] x [-2, TY+2] together with lam of -Ez at the corners around the owned
 // faces (one load latency for both), takes Up of cells [0, TX] x [0, TY] (the state
