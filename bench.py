@@ -321,6 +321,8 @@ def differentiable_grf(args, torch) -> dict:
     dts = np.stack(dts)
     badj = BatchAdjoint([(p.nx, p.ny, p.dx, p.gamma, p.bc) for p in probs], scheme)
     try:
+        # tile shapes of the forward and of both reverse passes, on one step
+        tiles, tiles_ms = badj.autotune(st0s, dts[:, :1], reps=2)
         badj.forward(st0s, dts[:, :1])             # warm-up (allocation, code load)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -347,6 +349,7 @@ def differentiable_grf(args, torch) -> dict:
     finally:
         badj.close()
     gnorm = [float(w.grad.norm()) for w in ws]
+    shape_name = ("16x16", "32x8")
     return {"workload": f"differentiable GRF ensemble: {len(sizes)} members, N in {{384,448,512}}, "
                         f"HLLD, MC, MUSCL+MOOD, CT, {steps} steps each (dt frozen), gradient wrt "
                         "the psi / Az noise fields; batched forward and adjoint (all members per "
@@ -355,6 +358,10 @@ def differentiable_grf(args, torch) -> dict:
             "forward_value": cells * steps / t_fwd,
             "fwd_bwd_value": cells * steps / (t2 - t0), "unit": "cell-updates/s",
             "cost_ratio": (t2 - t1) / t_fwd,
+            "tiles": {"forward": shape_name[tiles[0]], "reverse_a": shape_name[tiles[1]],
+                      "reverse_b": shape_name[tiles[2]],
+                      "autotune_ms": {"forward": [round(t, 3) for t in tiles_ms[:2]],
+                                      "reverse_ab": [round(t, 3) for t in tiles_ms[2:]]}},
             "all_grads_finite": bool(np.all(np.isfinite(gnorm)))}
 
 

@@ -240,10 +240,25 @@ class BatchAdjoint:
                                          d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
                                          int(seg_steps), _ptrs_many(lams)), "chimera_badj_sweep")
 
-    def set_tiles(self, shape_a: int = -1, shape_b: int = -1) -> None:
-        """Tile shapes of the two reverse passes: 0 = 16x16, 1 = 32x8."""
-        check(self._L.chimera_badj_set_tiles(self._h, int(shape_a), int(shape_b)),
+    def set_tiles(self, shape_a: int = -1, shape_b: int = -1, shape_f: int = -1) -> None:
+        """Tile shapes of the two reverse passes and of the forward stages: 0 = 16x16,
+        1 = 32x8 (-1 keeps)."""
+        check(self._L.chimera_badj_set_tiles(self._h, int(shape_a), int(shape_b), int(shape_f)),
               "chimera_badj_set_tiles")
+
+    def autotune(self, states, dts, reps: int = 2):
+        """Pick the fastest forward and reverse tile shapes on one step of every member
+        from states (dts: one dt per member).  Returns ((forward, A, B), ms) with ms the
+        mean times of forward shapes 0, 1 and of the (A, B) pairs (0,0), (0,1), (1,0),
+        (1,1)."""
+        d = np.ascontiguousarray(np.asarray(dts, dtype=np.float64).reshape(self.members, -1)[:, :1])
+        best = (C.c_int * 3)()
+        ms = (C.c_double * 6)()
+        torch.cuda.synchronize()
+        check(self._L.chimera_badj_autotune(self._h, _ptrs_many(states),
+                                            d.ctypes.data_as(C.POINTER(C.c_double)), int(reps),
+                                            best, ms), "chimera_badj_autotune")
+        return tuple(best), list(ms)
 
     def close(self) -> None:
         if getattr(self, "_h", None):

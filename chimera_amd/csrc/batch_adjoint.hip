@@ -183,8 +183,7 @@ __global__ __launch_bounds__(TX *TY) void badj_rev_b_kernel(const BAdjMember *__
     }
 }
 
-// tile shapes: forward 16 x 16; reverse 0 = 16 x 16, 1 = 32 x 8
-constexpr int FTX = 16, FTY = 16;
+// tile shapes of every pass (forward, reverse A, reverse B): 0 = 16 x 16, 1 = 32 x 8
 constexpr int BADJ_NUM_SHAPES = 2;
 constexpr int BADJ_SHAPES[BADJ_NUM_SHAPES][2] = {{16, 16}, {32, 8}};
 
@@ -192,16 +191,21 @@ using BFwdFn = void (*)(const BAdjMember *, const BTile *, int, int, int, int, i
 using BRevAFn = void (*)(const BAdjMember *, const BTile *, int, int, int);
 using BRevBFn = void (*)(const BAdjMember *, const BTile *, int, int, int);
 
-template <int S, bool PURE>
+template <int S, int TX, int TY, bool PURE>
 constexpr BFwdFn fwd_of(int limiter) {
-    return limiter == LIM_MC ? badj_forward_kernel<S, LIM_MC, FTX, FTY, PURE>
-                             : badj_forward_kernel<S, LIM_VAN_LEER, FTX, FTY, PURE>;
+    return limiter == LIM_MC ? badj_forward_kernel<S, LIM_MC, TX, TY, PURE>
+                             : badj_forward_kernel<S, LIM_VAN_LEER, TX, TY, PURE>;
+}
+template <int TX, int TY, bool PURE>
+BFwdFn fwd_shape(int solver, int limiter) {
+    return solver == RUSANOV ? fwd_of<RUSANOV, TX, TY, PURE>(limiter)
+         : solver == HLLE    ? fwd_of<HLLE, TX, TY, PURE>(limiter)
+                             : fwd_of<HLLD, TX, TY, PURE>(limiter);
 }
 template <bool PURE>
-BFwdFn fwd_fn(int solver, int limiter) {
-    return solver == RUSANOV ? fwd_of<RUSANOV, PURE>(limiter)
-         : solver == HLLE    ? fwd_of<HLLE, PURE>(limiter)
-                             : fwd_of<HLLD, PURE>(limiter);
+BFwdFn fwd_fn(int solver, int limiter, int shape) {
+    return shape == 0 ? fwd_shape<16, 16, PURE>(solver, limiter)
+                      : fwd_shape<32, 8, PURE>(solver, limiter);
 }
 
 template <int S, int TX, int TY>
@@ -244,9 +248,9 @@ struct chimera_badj {
     int traj_cap = 0;              // steps the trajectories hold
     double *dts = nullptr;         // members x dt_cap
     int dt_cap = 0;
-    BTile *dtiles[3] = {};         // forward, reverse shape 0, reverse shape 1
-    int ntiles[3] = {};
-    int shape_a = 1, shape_b = 1;
+    BTile *dtiles[BADJ_NUM_SHAPES] = {};   // tile tables of the shapes
+    int ntiles[BADJ_NUM_SHAPES] = {};
+    int shape_f = 0, shape_a = 1, shape_b = 1;   // forward, reverse A, reverse B
     int cu_count = 0;
     long long cells = 0;
     // the whole-run sweep: one member table per segment
@@ -269,13 +273,11 @@ int badj_tables(chimera_badj *b) {
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return b->hmem[x].n > b->hmem[y].n;
     });
-    const int shapes[3][2] = {{FTX, FTY}, {BADJ_SHAPES[0][0], BADJ_SHAPES[0][1]},
-                              {BADJ_SHAPES[1][0], BADJ_SHAPES[1][1]}};
-    for (int sh = 0; sh < 3; ++sh) {
+    for (int sh = 0; sh < BADJ_NUM_SHAPES; ++sh) {
         std::vector<BTile> t;
         for (int m : order) {
-            const int tx = (b->hmem[m].G.nx + shapes[sh][0] - 1) / shapes[sh][0];
-            const int ty = (b->hmem[m].G.ny + shapes[sh][1] - 1) / shapes[sh][1];
+            const int tx = (b->hmem[m].G.nx + BADJ_SHAPES[sh][0] - 1) / BADJ_SHAPES[sh][0];
+            const int ty = (b->hmem[m].G.ny + BADJ_SHAPES[sh][1] - 1) / BADJ_SHAPES[sh][1];
             for (int y = 0; y < ty; ++y)
                 for (int x = 0; x < tx; ++x) t.push_back({m, x, y});
         }
@@ -347,15 +349,15 @@ int badj_prepare(chimera_badj *b, const double *dt_in, int nsteps, int traj_step
 // state_out, keeping nothing; else the segment recompute, keeping every stage input
 // and every stage's Up in the trajectories.
 int badj_forward(chimera_badj *b, const BAdjMember *mem, int nsteps, bool pure) {
-    const BFwdFn fn = pure ? fwd_fn<true>(b->sch.riemann_solver, b->sch.slope_limiter)
-                           : fwd_fn<false>(b->sch.riemann_solver, b->sch.slope_limiter);
+    const BFwdFn fn = pure ? fwd_fn<true>(b->sch.riemann_solver, b->sch.slope_limiter, b->shape_f)
+                           : fwd_fn<false>(b->sch.riemann_solver, b->sch.slope_limiter, b->shape_f);
     int grid;
     CH_TRY(badj_grid(b, (const void *)fn, STAGE_THREADS, grid));
     for (int s = 0; s < nsteps; ++s)
         for (int stage = 1; stage <= 3; ++stage) {
             hipLaunchKernelGGL(fn, dim3(grid), dim3(STAGE_THREADS), 0, b->stream, mem,
-                               b->dtiles[0], b->ntiles[0], s, stage, b->sch.use_slope_limiting,
-                               b->sch.fallback_2_mood);
+                               b->dtiles[b->shape_f], b->ntiles[b->shape_f], s, stage,
+                               b->sch.use_slope_limiting, b->sch.fallback_2_mood);
             CH_TRY(launch_check("badj_forward_kernel"));
         }
     return CHIMERA_OK;
@@ -373,10 +375,10 @@ int badj_reverse(chimera_badj *b, const BAdjMember *mem, int nsteps) {
     CH_TRY(badj_grid(b, (const void *)fb, tb, gb));
     for (int s = nsteps - 1; s >= 0; --s)
         for (int st = 3; st >= 1; --st) {
-            hipLaunchKernelGGL(fa, dim3(ga), dim3(ta), 0, b->stream, mem,
-                               b->dtiles[1 + b->shape_a], b->ntiles[1 + b->shape_a], s, st);
-            hipLaunchKernelGGL(fb, dim3(gb), dim3(tb), 0, b->stream, mem,
-                               b->dtiles[1 + b->shape_b], b->ntiles[1 + b->shape_b], s, st);
+            hipLaunchKernelGGL(fa, dim3(ga), dim3(ta), 0, b->stream, mem, b->dtiles[b->shape_a],
+                               b->ntiles[b->shape_a], s, st);
+            hipLaunchKernelGGL(fb, dim3(gb), dim3(tb), 0, b->stream, mem, b->dtiles[b->shape_b],
+                               b->ntiles[b->shape_b], s, st);
             CH_TRY(launch_check("batched adjoint reverse stage"));
         }
     return CHIMERA_OK;
@@ -561,15 +563,103 @@ extern "C" int chimera_badj_sweep(chimera_badj_t *b, const double *const *states
     return CHIMERA_OK;
 }
 
-// Tile shapes of the two reverse passes: 0 = 16 x 16, 1 = 32 x 8 (-1 keeps).
-extern "C" int chimera_badj_set_tiles(chimera_badj_t *b, int shape_a, int shape_b) {
+// Tile shapes of the two reverse passes and of the forward stages: 0 = 16 x 16,
+// 1 = 32 x 8 (-1 keeps).
+extern "C" int chimera_badj_set_tiles(chimera_badj_t *b, int shape_a, int shape_b, int shape_f) {
     CH_REQUIRE(b, "NULL batched adjoint");
-    CH_REQUIRE(shape_a >= -1 && shape_a < BADJ_NUM_SHAPES && shape_b >= -1 &&
-                   shape_b < BADJ_NUM_SHAPES,
-               "tile shapes must be -1, 0 or 1, got %d, %d", shape_a, shape_b);
+    for (int v : {shape_a, shape_b, shape_f})
+        CH_REQUIRE(v >= -1 && v < BADJ_NUM_SHAPES, "tile shapes must be -1, 0 or 1, got %d", v);
     if (shape_a >= 0) b->shape_a = shape_a;
     if (shape_b >= 0) b->shape_b = shape_b;
+    if (shape_f >= 0) b->shape_f = shape_f;
     return CHIMERA_OK;
+}
+
+// Tile autotuner: times one forward step of every member from state0 (members x 6
+// device pointers, dts dt_in: members x 1, host) per forward shape, keeps the fastest,
+// recomputes that step, then times one reverse step per (A, B) shape pair and keeps the
+// fastest.  reps timed repetitions each after one untimed one.  Writes scratch arrays
+// of its own (the caller's are only read); best (may be NULL) gets the chosen
+// {forward, A, B} shapes, ms (may be NULL) the mean ms of the forward shapes 0, 1 and
+// of the pairs (A, B) = (0,0), (0,1), (1,0), (1,1).
+extern "C" int chimera_badj_autotune(chimera_badj_t *b, const double *const *state0,
+                                     const double *dt_in, int reps, int *best, double *ms) {
+    CH_REQUIRE(b && state0 && dt_in, "NULL argument");
+    CH_REQUIRE(reps >= 1, "reps must be >= 1");
+    for (int m = 0; m < b->members; ++m)
+        CH_REQUIRE(dt_in[m] > 0.0, "dt_in[%d] = %g is not positive", m, dt_in[m]);
+    CH_TRY(badj_prepare(b, dt_in, 1, 1));
+    // scratch: 6 arrays per member, the pure forward's output and then lam
+    size_t total = 0;
+    for (const auto &h : b->hmem) total += 6 * h.n;
+    double *scratch = nullptr;
+    CH_HIP(hipMalloc(&scratch, total * sizeof(double)));
+    std::vector<double *> sp((size_t)6 * b->members);
+    {
+        double *p = scratch;
+        for (int m = 0; m < b->members; ++m)
+            for (int q = 0; q < 6; ++q) {
+                sp[6 * (size_t)m + q] = p;
+                p += b->hmem[m].n;
+            }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = CHIMERA_OK;
+    auto timed = [&](auto &&body, double &out) -> int {
+        CH_TRY(body());   // untimed: code load, caches
+        CH_HIP(hipEventRecord(e0, b->stream));
+        for (int r = 0; r < reps; ++r) CH_TRY(body());
+        CH_HIP(hipEventRecord(e1, b->stream));
+        CH_HIP(hipEventSynchronize(e1));
+        float t = 0.0f;
+        CH_HIP(hipEventElapsedTime(&t, e0, e1));
+        out = (double)t / reps;
+        return CHIMERA_OK;
+    };
+    auto run = [&]() -> int {
+        CH_HIP(hipEventCreate(&e0));
+        CH_HIP(hipEventCreate(&e1));
+        CH_HIP(hipMemsetAsync(scratch, 0, total * sizeof(double), b->stream));
+        double t[2 + 2 * BADJ_NUM_SHAPES] = {};
+        CH_TRY(badj_upload(b, state0, nullptr, sp.data()));
+        for (int f = 0; f < BADJ_NUM_SHAPES; ++f) {
+            b->shape_f = f;
+            CH_TRY(timed([&] { return badj_forward(b, b->dmem, 1, true); }, t[f]));
+        }
+        b->shape_f = t[1] < t[0] ? 1 : 0;
+        CH_TRY(badj_upload(b, state0, sp.data(), nullptr));
+        CH_TRY(badj_forward(b, b->dmem, 1, false));
+        int ba = b->shape_a, bb = b->shape_b;
+        double tbest = 0.0;
+        for (int a = 0; a < BADJ_NUM_SHAPES; ++a)
+            for (int c = 0; c < BADJ_NUM_SHAPES; ++c) {
+                b->shape_a = a;
+                b->shape_b = c;
+                double &tt = t[2 + 2 * a + c];
+                CH_TRY(timed([&] { return badj_reverse(b, b->dmem, 1); }, tt));
+                if ((a == 0 && c == 0) || tt < tbest) {
+                    tbest = tt;
+                    ba = a;
+                    bb = c;
+                }
+            }
+        b->shape_a = ba;
+        b->shape_b = bb;
+        if (best) {
+            best[0] = b->shape_f;
+            best[1] = ba;
+            best[2] = bb;
+        }
+        if (ms)
+            for (int k = 0; k < 2 + 2 * BADJ_NUM_SHAPES; ++k) ms[k] = t[k];
+        return CHIMERA_OK;
+    };
+    rc = run();
+    (void)hipStreamSynchronize(b->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    return rc;
 }
 
 extern "C" void *chimera_badj_stream(chimera_badj_t *b) { return b ? (void *)b->stream : nullptr; }

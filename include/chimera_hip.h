@@ -330,8 +330,17 @@ int chimera_badj_segment_vjp(chimera_badj_t *badj, const double *const *state0,
  * counterpart in the reference (it has no derivatives). */
 int chimera_badj_sweep(chimera_badj_t *badj, const double *const *states, const double *dt_in,
                        int nsteps, int seg_steps, double *const *lam);
-/* Tile shapes of the two reverse passes: 0 = 16x16, 1 = 32x8 (default; -1 keeps). */
-int chimera_badj_set_tiles(chimera_badj_t *badj, int shape_a, int shape_b);
+/* Tile shapes of the two reverse passes (default 1) and of the forward stages
+ * (default 0): 0 = 16x16, 1 = 32x8 (-1 keeps).  Every shape gives the same bits. */
+int chimera_badj_set_tiles(chimera_badj_t *badj, int shape_a, int shape_b, int shape_f);
+/* Tile autotuner: times one forward step of every member from state0 (members x 6
+ * device pointers; dt_in: members x 1, host) per forward shape and one reverse step per
+ * (A, B) shape pair, reps times each after one untimed run, and keeps the fastest.
+ * Uses scratch arrays of its own.  best (may be NULL): the chosen {forward, A, B};
+ * ms (may be NULL): mean ms of forward shapes 0, 1, then of the pairs (0,0), (0,1),
+ * (1,0), (1,1).  No counterpart in the reference. */
+int chimera_badj_autotune(chimera_badj_t *badj, const double *const *state0, const double *dt_in,
+                          int reps, int *best, double *ms);
 void *chimera_badj_stream(chimera_badj_t *badj);
 
 #ifdef __cplusplus

@@ -321,7 +321,8 @@ def _member_dts(probs, nsteps):
 @pytest.mark.parametrize("scheme", [(1, 2, 1, 1), (2, 1, 1, 1), (3, 2, 1, 1)])
 def test_batched_adjoint_matches_single_grid(gpu_lib, scheme):
     """One persistent launch per stage over all members gives every member the bits of
-    the single-grid adjoint: forward steps and a segment VJP."""
+    the single-grid adjoint: forward steps and a segment VJP, with every tile shape and
+    after the tile autotuner has run."""
     torch = _torch()
     from chimera_amd.adjoint import AdjointStepper, BatchAdjoint
     probs = _batch_members(scheme)
@@ -338,6 +339,14 @@ def test_batched_adjoint_matches_single_grid(gpu_lib, scheme):
     badj.set_tiles(0, 0)
     lams2 = [l.clone() for l in lam0]
     badj.segment_vjp(x0, dts, lams2)
+    badj.set_tiles(1, 1, 1)   # 32x8 tiles in the forward stages too
+    fwd3 = badj.forward(x0, dts)
+    lams3 = [l.clone() for l in lam0]
+    badj.segment_vjp(x0, dts, lams3)
+    (fa, ra, rb), ms = badj.autotune(x0, dts[:, :1], reps=1)
+    assert {fa, ra, rb} <= {0, 1} and all(np.isfinite(ms)) and min(ms) > 0.0, (fa, ra, rb, ms)
+    lams4 = [l.clone() for l in lam0]
+    badj.segment_vjp(x0, dts, lams4)
     badj.close()
     for m, p in enumerate(probs):
         adj = AdjointStepper(p.nx, p.ny, p.dx, p.gamma, p.bc, p.scheme)
@@ -345,6 +354,7 @@ def test_batched_adjoint_matches_single_grid(gpu_lib, scheme):
         for s in range(nsteps):
             x = adj.forward_step(x, dts[m, s])
         assert np.array_equal(fwd[m].cpu().numpy(), x.cpu().numpy()), m
+        assert np.array_equal(fwd3[m].cpu().numpy(), x.cpu().numpy()), m
         lam = lam0[m].clone()
         adj.segment_vjp(x0[m], dts[m], lam)
         adj.close()
@@ -352,6 +362,8 @@ def test_batched_adjoint_matches_single_grid(gpu_lib, scheme):
         assert np.all(np.isfinite(ref))
         assert np.array_equal(lams[m].cpu().numpy(), ref), m
         assert np.array_equal(lams2[m].cpu().numpy(), ref), m
+        assert np.array_equal(lams3[m].cpu().numpy(), ref), m
+        assert np.array_equal(lams4[m].cpu().numpy(), ref), m
 
 
 def test_evolve_batch_gradient_matches_evolve(gpu_lib):
