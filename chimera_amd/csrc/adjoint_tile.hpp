@@ -431,20 +431,14 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
     }
 }
 
-// FPR fields per round, each round's loads issued one round ahead.  Two fields per
-// round shorten the field loop of a lone 512 x 256 grid by ~10%, but at 182 instead of
-// 146 VGPRs they leave no room for the recompute stream's stage kernels beside it,
-// which costs more in the pipelined sweep: one is the default.
-template <int TX, int TY, int FPR = 1>
+template <int TX, int TY>
 struct RevShapeB {
-    static_assert(FPR == 1 || FPR == 2 || FPR == 3, "6 fields in whole rounds");
-    static constexpr int ROUNDS = 6 / FPR;
     static constexpr int NT = TX * TY;
     static constexpr int PW = TX + 5, PH = TY + 5, P_SZ = PW * PH;   // primitive: cells [-2, T+2]
     static constexpr int GW = TX + 3, GH = TY + 3, G_SZ = GW * GH;   // slope parts: cells [-1, T+1]
     static constexpr int CW = TX + 1, CH = TY + 1, C_SZ = CW * CH;   // cells [0, T]
-    // the round's primitives and slope parts, lam of cell B, lam of the owned cells' U
-    static constexpr int LDS_DOUBLES = FPR * (P_SZ + 5 * G_SZ) + 2 * C_SZ + 4 * NT;
+    // one field's primitive and slope parts, lam of cell B, lam of the owned cells' U
+    static constexpr int LDS_DOUBLES = P_SZ + 5 * G_SZ + 2 * C_SZ + 4 * NT;
     static constexpr int CPT = (C_SZ + NT - 1) / NT;                 // cells [0, T] per thread
 };
 
@@ -452,19 +446,19 @@ struct RevShapeB {
 // EARLY_TAIL issues the tail's loads with the last round (a lone small grid, whose few
 // waves per SIMD cannot hide their latency); off, they are issued where they are used,
 // which keeps 56 VGPRs out of the field loop for grids with waves enough to hide it.
-template <int LIMITER, int TX, int TY, int FPR = 1, bool EARLY_TAIL = true>
+template <int LIMITER, int TX, int TY, bool EARLY_TAIL = true>
 __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const *U,
                                            const FusedBufs &F, double *const *lin,
                                            const double *const *lU0_add, int tile_x, int tile_y,
                                            double *lds) {
-    using B = RevShapeB<TX, TY, FPR>;
+    using B = RevShapeB<TX, TY>;
     constexpr int NT = B::NT;
     const int tid = threadIdx.x;
     const int nx = G.nx, ny = G.ny, px = G.px, py = G.py;
     const int x0 = tile_x * TX, y0 = tile_y * TY;
-    double *sA = lds;                       // [FPR] primitives, cells [-2, T+2]
-    double *sG = sA + FPR * B::P_SZ;        // [FPR] gmx, gpx, gmy, gpy, lP, cells [-1, T+1]
-    double *sB = sG + FPR * 5 * B::G_SZ;    // lam of cell B (total), cells [0, T]
+    double *sA = lds;                       // one primitive, cells [-2, T+2]
+    double *sG = sA + B::P_SZ;              // gmx, gpx, gmy, gpy, lP of one field, cells [-1, T+1]
+    double *sB = sG + 5 * B::G_SZ;          // lam of cell B (total), cells [0, T]
     double *sU = sB + 2 * B::C_SZ;          // lam of the owned cells' conserved U (cons_to_prim)
     const double hdx = 0.5 * G.dx, inv_dx = 1.0 / G.dx, inv_2dx = 1.0 / (2.0 * G.dx);
     // valid global cell of a local position, or -1 outside a non-periodic grid
@@ -478,7 +472,7 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
     double lp6[B::CPT][6];
 
     // per-thread positions of the primitive items and of the slope items, fixed over the
-    // six fields; each round's loads are issued one round ahead of its use
+    // six fields; each field's loads are issued one round ahead of its use
     constexpr int PPT = (B::P_SZ + NT - 1) / NT, GPT = (B::G_SZ + NT - 1) / NT;
     size_t kp[PPT];
 #pragma unroll
@@ -509,25 +503,21 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
         t.cym = !py && j == 0;
         t.cyp = !py && j == ny - 1;
     }
-    double pv[2][FPR][PPT], rv[2][FPR][GPT][6];
-    auto load = [&](int r, int buf) {
+    double pv[2][PPT], rv[2][GPT][6];
+    auto load = [&](int f, int buf) {
 #pragma unroll
-        for (int h = 0; h < FPR; ++h) {
-            const int f = r * FPR + h;
+        for (int u = 0; u < PPT; ++u)
+            if (tid + u * NT < B::P_SZ) pv[buf][u] = F.P(f)[kp[u]];
 #pragma unroll
-            for (int u = 0; u < PPT; ++u)
-                if (tid + u * NT < B::P_SZ) pv[buf][h][u] = F.P(f)[kp[u]];
-#pragma unroll
-            for (int u = 0; u < GPT; ++u) {
-                const GItem &t = gt[u];
-                if (!t.valid) continue;
-                rv[buf][h][u][0] = F.XL(f)[t.k];
-                rv[buf][h][u][1] = t.hl ? F.XR(f)[t.kl] : 0.0;   // right state of face i-1/2
-                rv[buf][h][u][2] = F.YL(f)[t.k];
-                rv[buf][h][u][3] = t.hb ? F.YR(f)[t.kb] : 0.0;
-                rv[buf][h][u][4] = F.Pr(f)[t.k];
-                rv[buf][h][u][5] = (t.cxm || t.cym) ? F.Pd(f)[t.k] : 0.0;   // open low side
-            }
+        for (int u = 0; u < GPT; ++u) {
+            const GItem &t = gt[u];
+            if (!t.valid) continue;
+            rv[buf][u][0] = F.XL(f)[t.k];
+            rv[buf][u][1] = t.hl ? F.XR(f)[t.kl] : 0.0;   // right state of face i-1/2
+            rv[buf][u][2] = F.YL(f)[t.k];
+            rv[buf][u][3] = t.hb ? F.YR(f)[t.kb] : 0.0;
+            rv[buf][u][4] = F.Pr(f)[t.k];
+            rv[buf][u][5] = (t.cxm || t.cym) ? F.Pd(f)[t.k] : 0.0;   // open low side
         }
     };
     // the tail's inputs: per cell of [0, T] its B, lam of its cell B and its conserved U;
@@ -569,93 +559,83 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
     load(0, 0);
 
 #pragma unroll
-    for (int r = 0; r < B::ROUNDS; ++r) {
-        const int cur = r & 1;
-        // the round's stage primitives over cells [-2, T+2] (clamped like the forward)
+    for (int f = 0; f < 6; ++f) {
+        const int cur = f & 1;
+        // the field's stage primitive over cells [-2, T+2] (clamped like the forward)
 #pragma unroll
-        for (int h = 0; h < FPR; ++h)
-#pragma unroll
-            for (int u = 0; u < PPT; ++u)
-                if (tid + u * NT < B::P_SZ) sA[h * B::P_SZ + tid + u * NT] = pv[cur][h][u];
+        for (int u = 0; u < PPT; ++u)
+            if (tid + u * NT < B::P_SZ) sA[tid + u * NT] = pv[cur][u];
         __syncthreads();
-        if (r + 1 < B::ROUNDS) load(r + 1, cur ^ 1);
-        else if (EARLY_TAIL) {
+        if (f < 5) {
+            load(f + 1, cur ^ 1);
+        } else if (EARLY_TAIL) {
             load_cells();
             load_owned();
         }
         // slope parts of cells [-1, T+1] (as ar_slopes)
 #pragma unroll
-        for (int h = 0; h < FPR; ++h) {
-            const double *sAh = sA + h * B::P_SZ;
-            double *sGh = sG + h * 5 * B::G_SZ;
+        for (int u = 0; u < GPT; ++u) {
+            const int idx = tid + u * NT;
+            if (idx >= B::G_SZ) break;
+            const GItem &t = gt[u];
+            if (!t.valid) {
 #pragma unroll
-            for (int u = 0; u < GPT; ++u) {
-                const int idx = tid + u * NT;
-                if (idx >= B::G_SZ) break;
-                const GItem &t = gt[u];
-                if (!t.valid) {
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) sGh[c * B::G_SZ + idx] = 0.0;
-                    continue;
-                }
-                const int lx = idx % B::GW - 1, ly = idx / B::GW - 1;
-                const double *a = sAh + (ly + 2) * B::PW + (lx + 2);
-                const double am = t.cxm ? a[0] : a[-1], ap = t.cxp ? a[0] : a[1];
-                const double bm = t.cym ? a[0] : a[-B::PW], bp = t.cyp ? a[0] : a[B::PW];
-                const double *r6 = rv[cur][h][u];
-                const double xr = r6[1], yr = r6[3];
-                const double lSx = hdx * (r6[0] - xr);
-                const double lSy = hdx * (r6[2] - yr);
-                double lp = r6[0] + xr + r6[2] + yr + r6[4] + r6[5];
-                double dm, d0, dp;
-                if (G.use_lim) limited_slope_grad<LIMITER>(am, a[0], ap, inv_dx, dm, d0, dp);
-                else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
-                double gm = lSx * dm, gp = lSx * dp;
-                lp += lSx * d0;
-                if (t.cxm) { lp += gm; gm = 0.0; }
-                if (t.cxp) { lp += gp; gp = 0.0; }
-                sGh[0 * B::G_SZ + idx] = gm;
-                sGh[1 * B::G_SZ + idx] = gp;
-                if (G.use_lim) limited_slope_grad<LIMITER>(bm, a[0], bp, inv_dx, dm, d0, dp);
-                else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
-                gm = lSy * dm;
-                gp = lSy * dp;
-                lp += lSy * d0;
-                if (t.cym) { lp += gm; gm = 0.0; }
-                if (t.cyp) { lp += gp; gp = 0.0; }
-                sGh[2 * B::G_SZ + idx] = gm;
-                sGh[3 * B::G_SZ + idx] = gp;
-                sGh[4 * B::G_SZ + idx] = lp;
+                for (int c = 0; c < 5; ++c) sG[c * B::G_SZ + idx] = 0.0;
+                continue;
             }
+            const int lx = idx % B::GW - 1, ly = idx / B::GW - 1;
+            const double *a = sA + (ly + 2) * B::PW + (lx + 2);
+            const double am = t.cxm ? a[0] : a[-1], ap = t.cxp ? a[0] : a[1];
+            const double bm = t.cym ? a[0] : a[-B::PW], bp = t.cyp ? a[0] : a[B::PW];
+            const double *r6 = rv[cur][u];
+            const double xr = r6[1], yr = r6[3];
+            const double lSx = hdx * (r6[0] - xr);
+            const double lSy = hdx * (r6[2] - yr);
+            double lp = r6[0] + xr + r6[2] + yr + r6[4] + r6[5];
+            double dm, d0, dp;
+            if (G.use_lim) limited_slope_grad<LIMITER>(am, a[0], ap, inv_dx, dm, d0, dp);
+            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
+            double gm = lSx * dm, gp = lSx * dp;
+            lp += lSx * d0;
+            if (t.cxm) { lp += gm; gm = 0.0; }
+            if (t.cxp) { lp += gp; gp = 0.0; }
+            sG[0 * B::G_SZ + idx] = gm;
+            sG[1 * B::G_SZ + idx] = gp;
+            if (G.use_lim) limited_slope_grad<LIMITER>(bm, a[0], bp, inv_dx, dm, d0, dp);
+            else { dm = -inv_2dx; d0 = 0.0; dp = inv_2dx; }
+            gm = lSy * dm;
+            gp = lSy * dp;
+            lp += lSy * d0;
+            if (t.cym) { lp += gm; gm = 0.0; }
+            if (t.cyp) { lp += gp; gp = 0.0; }
+            sG[2 * B::G_SZ + idx] = gm;
+            sG[3 * B::G_SZ + idx] = gp;
+            sG[4 * B::G_SZ + idx] = lp;
         }
         __syncthreads();
         // own part plus the neighbours' parts for cells [0, T] (as ar_prims)
 #pragma unroll
-        for (int h = 0; h < FPR; ++h) {
-            const double *sGh = sG + h * 5 * B::G_SZ;
-#pragma unroll
-            for (int u = 0; u < B::CPT; ++u) {
-                const int c = tid + u * NT;
-                if (c >= B::C_SZ) break;
-                const int lx = c % B::CW, ly = c / B::CW;
-                int i, j;
-                if (!gcell(lx, ly, i, j)) {
-                    lp6[u][r * FPR + h] = 0.0;
-                    continue;
-                }
-                const bool hl = i > 0 || px, hr = i < nx - 1 || px;
-                const bool hb = j > 0 || py, ht = j < ny - 1 || py;
-                const int gidx = (ly + 1) * B::GW + (lx + 1);
-                double acc = sGh[4 * B::G_SZ + gidx];
-                if (hl) acc += sGh[1 * B::G_SZ + gidx - 1];
-                if (hr) acc += sGh[0 * B::G_SZ + gidx + 1];
-                if (hb) acc += sGh[3 * B::G_SZ + gidx - B::GW];
-                if (ht) acc += sGh[2 * B::G_SZ + gidx + B::GW];
-                lp6[u][r * FPR + h] = acc;
+        for (int u = 0; u < B::CPT; ++u) {
+            const int c = tid + u * NT;
+            if (c >= B::C_SZ) break;
+            const int lx = c % B::CW, ly = c / B::CW;
+            int i, j;
+            if (!gcell(lx, ly, i, j)) {
+                lp6[u][f] = 0.0;
+                continue;
             }
+            const bool hl = i > 0 || px, hr = i < nx - 1 || px;
+            const bool hb = j > 0 || py, ht = j < ny - 1 || py;
+            const int gidx = (ly + 1) * B::GW + (lx + 1);
+            double acc = sG[4 * B::G_SZ + gidx];
+            if (hl) acc += sG[1 * B::G_SZ + gidx - 1];
+            if (hr) acc += sG[0 * B::G_SZ + gidx + 1];
+            if (hb) acc += sG[3 * B::G_SZ + gidx - B::GW];
+            if (ht) acc += sG[2 * B::G_SZ + gidx + B::GW];
+            lp6[u][f] = acc;
         }
-        // the next round overwrites sA and sG; the tail uses sB and sU only
-        if (r + 1 < B::ROUNDS) __syncthreads();
+        // the next field overwrites sA and sG; the tail uses sB and sU only
+        if (f < 5) __syncthreads();
     }
     // reverse of cons_to_prim for cells [0, T]; lam of the owned cells' U into sU
 #pragma unroll

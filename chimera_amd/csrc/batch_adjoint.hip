@@ -177,7 +177,7 @@ __global__ __launch_bounds__(TX *TY) void badj_rev_b_kernel(const BAdjMember *__
             U[q] = stage_in(m, s, st, q);
             lu0[q] = m.lam[2][q];
         }
-        rev_tile_b<LIMITER, TX, TY, 1, false>(m.G, U, m.F, lin, st == 1 ? lu0 : nullptr, tr.tx,
+        rev_tile_b<LIMITER, TX, TY, false>(m.G, U, m.F, lin, st == 1 ? lu0 : nullptr, tr.tx,
                                                tr.ty, lds);
         __syncthreads();
     }
