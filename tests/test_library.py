@@ -86,6 +86,26 @@ def test_invalid_arguments_rejected_on_host(L):
                                       C.byref(BC(1, 1, 1, 1))) == -1
 
 
+def test_adjoint_entry_points_reject_invalid_arguments_on_host(L):
+    """The sweep, batched-sweep, autotune and tile entry points validate their
+    arguments before touching the device."""
+    from chimera_amd._lib import BC, Scheme
+    ptrs = (C.c_void_p * 6)()
+    dts = (C.c_double * 4)(1e-3, 1e-3, 1e-3, 1e-3)
+    assert L.chimera_adj_sweep(None, ptrs, dts, 4, 2, ptrs) == -1
+    assert b"NULL" in L.chimera_last_error()
+    assert L.chimera_badj_sweep(None, ptrs, dts, 4, 2, ptrs) == -1
+    assert L.chimera_badj_autotune(None, ptrs, dts, 1, None, None) == -1
+    assert L.chimera_badj_set_tiles(None, 0, 0, 0) == -1
+    h = C.c_void_p()
+    n = (C.c_int * 1)(8)
+    d = (C.c_double * 1)(0.1)
+    g = (C.c_double * 1)(1.4)
+    assert L.chimera_badj_create(C.byref(h), 0, n, n, d, g, C.byref(BC(1, 1, 1, 1)),
+                                 C.byref(Scheme(3, 2, 1, 1)), None) == -1
+    assert b"members" in L.chimera_last_error()
+
+
 def test_scheme_and_bc_structs():
     from chimera_amd._lib import BC, Scheme
     assert C.sizeof(BC) == 16 and C.sizeof(Scheme) == 16
