@@ -85,22 +85,26 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// max over cells of cf + |v| of the primitives of state u (src/main.f90:69-79).
+// max over cells of cf + |v| of the primitives of state u (src/main.f90:69-79).  Blocks
+// stride over rows and threads over a row's cells (coalesced, no index division); the
+// max is exact, so the order of the reduction does not change dt.
 __global__ __launch_bounds__(256) void cfl_reduce_kernel(CflArgs a) {
     __shared__ double part[4];
-    const size_t n = (size_t)a.nx * a.ny;
+    const int nx = a.nx, ny = a.ny;
     double s = 0.0;
-    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < n;
-         idx += (size_t)gridDim.x * 256) {
-        int i = (int)(idx % a.nx), j = (int)(idx / a.nx);
-        int im = i > 0 ? i - 1 : (a.px ? a.nx - 1 : 0);
-        int jm = j > 0 ? j - 1 : (a.py ? a.ny - 1 : 0);
-        double Bx = 0.5 * (a.u[4][idx] + a.u[4][(size_t)im + (size_t)a.nx * j]);
-        double By = 0.5 * (a.u[5][idx] + a.u[5][(size_t)i + (size_t)a.nx * jm]);
-        double rho, vx, vy, P;
-        cons_to_prim(a.u[0][idx], a.u[1][idx], a.u[2][idx], a.u[3][idx], Bx, By, a.gamma, a.vol,
-                     rho, vx, vy, P);
-        s = fmax(s, cfl_speed(rho, vx, vy, P, Bx, By, a.gamma));
+    for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+        const int jm = j > 0 ? j - 1 : (a.py ? ny - 1 : 0);
+        const size_t row = (size_t)nx * j, rowm = (size_t)nx * jm;
+        for (int i = threadIdx.x; i < nx; i += 256) {
+            const int im = i > 0 ? i - 1 : (a.px ? nx - 1 : 0);
+            const size_t idx = row + i;
+            double Bx = 0.5 * (a.u[4][idx] + a.u[4][row + im]);
+            double By = 0.5 * (a.u[5][idx] + a.u[5][rowm + i]);
+            double rho, vx, vy, P;
+            cons_to_prim(a.u[0][idx], a.u[1][idx], a.u[2][idx], a.u[3][idx], Bx, By, a.gamma,
+                         a.vol, rho, vx, vy, P);
+            s = fmax(s, cfl_speed(rho, vx, vy, P, Bx, By, a.gamma));
+        }
     }
     s = wave_max(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
@@ -198,8 +202,7 @@ int launch_stage(const StageLaunch &L, hipStream_t stream) {
 }
 
 int launch_cfl(const CflArgs &a, hipStream_t stream) {
-    size_t n = (size_t)a.nx * a.ny;
-    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    const int blocks = std::max(1, std::min(a.ny, 4096));   // one row per block and round
     hipLaunchKernelGGL(cfl_reduce_kernel, dim3(blocks), dim3(256), 0, stream, a);
     return launch_check("cfl_reduce_kernel");
 }
