@@ -115,7 +115,9 @@ __global__ __launch_bounds__(256) void cfl_reduce_kernel(CflArgs a) {
     }
 }
 
-// dt of the step (src/main.f90:79-83), or the prescribed value.
+// dt of the step (src/main.f90:79-83), or the prescribed value, then the step's advance
+// of t and of the output counter (src/main.f90:273-292) in the same thread: the stages
+// read only dt, so t can move before they run.
 __global__ void dt_finalize_kernel(StepState *st, int mode, double dx, const double *dt_seq) {
     double dt;
     if (mode == CHIMERA_DT_PRESCRIBED) {
@@ -131,11 +133,7 @@ __global__ void dt_finalize_kernel(StepState *st, int mode, double dx, const dou
         dt = fmax(dt, (double)1e-12f);  // single-precision literal in the reference
     }
     st->dt = dt;
-}
-
-// t += dt and the output counter (src/main.f90:273-292).
-__global__ void step_end_kernel(StepState *st) {
-    double t = st->t + st->dt;
+    const double t = st->t + dt;
     st->t = t;
     if (t >= st->out_count * T_OUT) st->out_count = st->out_count + 1;
     st->step = st->step + 1;
@@ -211,11 +209,6 @@ int launch_dt_finalize(StepState *st, int mode, double dx, const double *dt_seq,
                        hipStream_t stream) {
     hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, stream, st, mode, dx, dt_seq);
     return launch_check("dt_finalize_kernel");
-}
-
-int launch_step_end(StepState *st, hipStream_t stream) {
-    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(1), 0, stream, st);
-    return launch_check("step_end_kernel");
 }
 
 int launch_set_state(StepState *st, double t, int oc, int step, hipStream_t stream) {
@@ -386,7 +379,6 @@ static int enqueue_step(chimera_ctx *c, int mode, bool timed) {
             CH_TRY(launch_stage(L, c->stream));
         }
     }
-    CH_TRY(launch_step_end(c->st, c->stream));
     return CHIMERA_OK;
 }
 

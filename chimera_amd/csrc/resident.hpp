@@ -52,7 +52,6 @@ int launch_stage(const StageLaunch &L, hipStream_t stream);
 int launch_cfl(const CflArgs &a, hipStream_t stream);
 int launch_dt_finalize(StepState *st, int mode, double dx, const double *dt_seq,
                        hipStream_t stream);
-int launch_step_end(StepState *st, hipStream_t stream);
 int launch_set_state(StepState *st, double t, int oc, int step, hipStream_t stream);
 int launch_zero_u32(uint32_t *p, size_t n, hipStream_t stream);
 int launch_divb(const double *bx, const double *by, int nx, int ny, int px, int py, double dx,
