@@ -132,6 +132,7 @@ def _declare(L: C.CDLL) -> None:
         "chimera_adj_step_vjp": [vp, C.POINTER(vp), d, C.POINTER(vp), C.POINTER(vp)],
         "chimera_adj_segment_vjp": [vp, C.POINTER(vp), c_dp, i, C.POINTER(vp)],
         "chimera_adj_sweep": [vp, C.POINTER(vp), c_dp, i, i, C.POINTER(vp)],
+        "chimera_adj_record": [vp, C.POINTER(vp), c_dp, i, i, i, C.POINTER(vp)],
         "chimera_adj_set_fused": [vp, i],
         "chimera_adj_set_tiles": [vp, i, i],
         "chimera_face_vjp": [i, i, i, C.c_double, c_dp, c_dp, c_dp],

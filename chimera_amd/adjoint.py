@@ -89,6 +89,18 @@ class AdjointStepper:
                                         int(seg_steps), _ptrs(lam)), "chimera_adj_sweep")
         return lam
 
+    def record(self, x0: torch.Tensor, dts, seg_steps: int, nseg: int,
+               out: torch.Tensor) -> torch.Tensor:
+        """len(dts) (<= seg_steps) steps from x0 into out, keeping the trajectory where
+        the sweep of a run of nseg segments checkpointed every seg_steps steps expects its
+        last segment; that sweep (with x0 as its last checkpoint and the same dts) then
+        skips the segment's recompute."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        check(self._L.chimera_adj_record(self._h, _ptrs(x0), dts.ctypes.data_as(C.POINTER(C.c_double)),
+                                         len(dts), int(seg_steps), int(nseg), _ptrs(out)),
+              "chimera_adj_record")
+        return out
+
     def close(self):
         if getattr(self, "_h", None):
             self._L.chimera_adj_destroy(self._h)
@@ -140,15 +152,22 @@ class _Evolve(torch.autograd.Function):
         U0 = U0.detach().contiguous()
         n = len(dts)
         K = max(1, int(checkpoint_every))
+        nseg = -(-n // K)
+        s_last = (nseg - 1) * K
+        # with a backward to follow, the last segment runs through the adjoint workspace,
+        # which keeps its trajectory for the sweep (no recompute of that segment)
+        record = ctx.needs_input_grad[0]
         ckpts = [U0.clone()]
         with Grid(nx, ny, dx, gamma, bc, scheme) as g:
             g.upload(U0)
-            for s0 in range(0, n, K):
-                k = min(K, n - s0)
-                g.steps(k, dt_mode=DT_PRESCRIBED, dt=dts[s0:s0 + k])
-                if s0 + k < n:
-                    ckpts.append(g.download(torch.empty_like(U0)))
-            UN = g.download(torch.empty_like(U0))
+            for s0 in range(0, s_last, K):
+                g.steps(K, dt_mode=DT_PRESCRIBED, dt=dts[s0:s0 + K])
+                ckpts.append(g.download(torch.empty_like(U0)))
+            if not record:
+                g.steps(n - s_last, dt_mode=DT_PRESCRIBED, dt=dts[s_last:])
+                UN = g.download(torch.empty_like(U0))
+        if record:
+            UN = _workspace(geom).record(ckpts[-1], dts[s_last:], K, nseg, torch.empty_like(U0))
         ctx.save_for_backward(*ckpts)
         ctx.dts, ctx.geom, ctx.K = dts, geom, K
         return UN

@@ -446,6 +446,13 @@ struct chimera_adj {
     hipEvent_t ev_start = nullptr, ev_traj[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     double **ck_tab = nullptr;    // device tables of the checkpoints, 6 pointers each
     int ck_cap = 0;
+    // the last segment of a run recorded by chimera_adj_record for the next sweep
+    struct Recorded {
+        bool valid = false;
+        const double *state0[6] = {};
+        int nsteps = 0, K = 0, nseg = 0;
+        std::vector<double> dts;
+    } rec;
 };
 
 namespace {
@@ -680,6 +687,7 @@ int vjp_core(chimera_adj *a, const Ptr6 &hs, Tab ts, const double *dt_dev, doubl
 // pointers per slot (grown on demand, kept)
 int ensure_slots(chimera_adj *a, double *&buf, double **&tab, int &cap, int slots, int fields) {
     if (slots <= cap) return CHIMERA_OK;
+    a->rec.valid = false;   // a recorded segment does not survive a reallocation
     CH_HIP(hipStreamSynchronize(a->stream));
     if (a->rstream) CH_HIP(hipStreamSynchronize(a->rstream));
     (void)hipFree(buf);
@@ -775,6 +783,21 @@ int finish_lam(chimera_adj *a, Tab cur, double *const *lam) {
             CH_HIP(hipMemcpyAsync(lam[q], a->lam_c.p[q], bytes, hipMemcpyDeviceToDevice, a->stream));
     }
     return CHIMERA_OK;
+}
+
+// what a sweep needs of a recorded last segment
+struct Recorded_check_t {
+    int nseg, K, nsteps;
+    const double *const *state0;
+    const double *dts;
+};
+
+bool recorded_matches(const chimera_adj *a, const Recorded_check_t &c) {
+    const auto &r = a->rec;
+    if (!r.valid || r.nseg != c.nseg || r.K != c.K || r.nsteps != c.nsteps) return false;
+    for (int q = 0; q < 6; ++q)
+        if (r.state0[q] != c.state0[q]) return false;
+    return std::equal(r.dts.begin(), r.dts.end(), c.dts);
 }
 
 int ensure_sweep(chimera_adj *a, int nseg) {
@@ -964,6 +987,7 @@ extern "C" int chimera_adj_segment_vjp(chimera_adj_t *a, const double *const *st
     CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
     for (int s = 0; s < nsteps; ++s)
         CH_REQUIRE(dt_in[s] > 0.0, "dt_in[%d] = %g is not positive", s, dt_in[s]);
+    a->rec.valid = false;   // this call overwrites the trajectory
     CH_TRY(ensure_traj(a, nsteps, 1));
     CH_TRY(set_dts(a, dt_in, nsteps));
     CH_TRY(set_table(a, TB_IN, state0));
@@ -978,6 +1002,36 @@ extern "C" int chimera_adj_segment_vjp(chimera_adj_t *a, const double *const *st
     return CHIMERA_OK;
 }
 
+extern "C" int chimera_adj_record(chimera_adj_t *a, const double *const *state0,
+                                  const double *dt_in, int nsteps, int seg_steps, int nseg,
+                                  double *const *state_out) {
+    CH_REQUIRE(a && state0 && dt_in && state_out, "NULL argument");
+    CH_REQUIRE(nsteps >= 1 && nseg >= 1 && seg_steps >= nsteps,
+               "need 1 <= nsteps <= seg_steps and nseg >= 1 (got %d, %d, %d)", nsteps, seg_steps,
+               nseg);
+    for (int s = 0; s < nsteps; ++s)
+        CH_REQUIRE(dt_in[s] > 0.0, "dt_in[%d] = %g is not positive", s, dt_in[s]);
+    a->rec.valid = false;
+    // the layout chimera_adj_sweep uses for segment nseg - 1 of this run
+    const int K = nseg > 1 ? seg_steps : nsteps, h = (nseg - 1) & 1;
+    CH_TRY(ensure_traj(a, K, nseg > 1 ? 2 : 1));
+    CH_TRY(set_dts(a, dt_in, nsteps));
+    CH_TRY(recompute_segment(a, a->stream, ptr6(state0), a->dt_dev, nsteps, h * (3 * K + 1),
+                             h * 3 * K));
+    const Ptr6 last = traj_slot(a, h * (3 * K + 1) + 3 * nsteps);
+    for (int q = 0; q < 6; ++q)
+        CH_HIP(hipMemcpyAsync(state_out[q], last.p[q], a->n * sizeof(double),
+                              hipMemcpyDeviceToDevice, a->stream));
+    CH_HIP(hipStreamSynchronize(a->stream));
+    a->rec.valid = true;
+    for (int q = 0; q < 6; ++q) a->rec.state0[q] = state0[q];
+    a->rec.nsteps = nsteps;
+    a->rec.K = K;
+    a->rec.nseg = nseg;
+    a->rec.dts.assign(dt_in, dt_in + nsteps);
+    return CHIMERA_OK;
+}
+
 extern "C" int chimera_adj_sweep(chimera_adj_t *a, const double *const *states, const double *dt_in,
                                  int nsteps, int seg_steps, double *const *lam) {
     CH_REQUIRE(a && states && dt_in && lam, "NULL argument");
@@ -987,16 +1041,22 @@ extern "C" int chimera_adj_sweep(chimera_adj_t *a, const double *const *states, 
     const int nseg = (nsteps + seg_steps - 1) / seg_steps;
     for (int k = 0; k < 6 * nseg; ++k) CH_REQUIRE(states[k], "states[%d] is NULL", k);
     const int K = std::min(seg_steps, nsteps);
+    auto seg_len = [&](int k) { return std::min(seg_steps, nsteps - k * seg_steps); };
     // one segment: the trajectory halves alternate, so two are enough
     CH_TRY(ensure_traj(a, K, nseg > 1 ? 2 : 1));
     CH_TRY(ensure_sweep(a, nseg));
+    // the last segment recorded by chimera_adj_record for exactly this run (checked after
+    // the allocations, which drop a recorded segment when they reallocate)?
+    const Recorded_check_t rc_last = {nseg, K, seg_len(nseg - 1), states + 6 * (size_t)(nseg - 1),
+                                      dt_in + (size_t)(nseg - 1) * seg_steps};
+    const bool have_last = recorded_matches(a, rc_last);
+    a->rec.valid = false;
     CH_TRY(set_dts(a, dt_in, nsteps));
     CH_TRY(set_table(a, TB_LIN, (const double *const *)lam));
     CH_HIP(hipMemcpy(a->ck_tab, states, (size_t)nseg * 6 * sizeof(double *), hipMemcpyHostToDevice));
     // the recompute stream starts after everything queued on the main stream
     CH_HIP(hipEventRecord(a->ev_start, a->stream));
     CH_HIP(hipStreamWaitEvent(a->rstream, a->ev_start, 0));
-    auto seg_len = [&](int k) { return std::min(seg_steps, nsteps - k * seg_steps); };
     auto recompute = [&](int k) -> int {
         const int b = k & 1;
         CH_TRY(recompute_segment(a, a->rstream, ptr6(states + 6 * (size_t)k),
@@ -1006,7 +1066,11 @@ extern "C" int chimera_adj_sweep(chimera_adj_t *a, const double *const *states, 
         return CHIMERA_OK;
     };
     Tab cur = table(a, TB_LIN), other = table(a, TB_LC);
-    CH_TRY(recompute(nseg - 1));
+    if (have_last) {
+        CH_HIP(hipEventRecord(a->ev_traj[(nseg - 1) & 1], a->rstream));   // already there
+    } else {
+        CH_TRY(recompute(nseg - 1));
+    }
     for (int k = nseg - 1; k >= 0; --k) {
         if (k >= 1) {
             // segment k-1 goes into the half segment k+1 used: after that reverse

@@ -279,6 +279,15 @@ int chimera_adj_segment_vjp(chimera_adj_t *adj, const double *const *state0, con
  * autograd backward; no counterpart in the reference (it has no derivatives). */
 int chimera_adj_sweep(chimera_adj_t *adj, const double *const *states, const double *dt_in,
                       int nsteps, int seg_steps, double *const *lam);
+/* Forward of the last checkpoint segment of a run the next chimera_adj_sweep reverses:
+ * nsteps (<= seg_steps) steps from state0 (device) with the dts dt_in (host), keeping
+ * the segment's trajectory where that sweep (a run of nseg segments checkpointed every
+ * seg_steps steps) looks for it; the final state goes to state_out (device).  A sweep
+ * whose last segment has the same state0 pointers, dts, seg_steps and nseg skips that
+ * segment's recompute; any other trajectory use of the workspace drops the recording.
+ * No counterpart in the reference. */
+int chimera_adj_record(chimera_adj_t *adj, const double *const *state0, const double *dt_in,
+                       int nsteps, int seg_steps, int nseg, double *const *state_out);
 void *chimera_adj_stream(chimera_adj_t *adj);
 
 /* Which reverse kernels the adjoint runs: 1 (default) the fused LDS-tiled pair that

@@ -520,3 +520,31 @@ def test_grf_noise_gradient_vs_finite_differences(gpu_lib):
     print(f"GRF noise gradient: adjoint {dd:.12e}, FD {fd:.12e}", flush=True)
     assert abs(dd) > 0.0
     assert abs(fd - dd) <= 1.0e-6 * abs(dd), (fd, dd)
+
+
+@pytest.mark.parametrize("nsteps,K", [(23, 5), (6, 10)])
+def test_sweep_reuses_recorded_last_segment(gpu_lib, nsteps, K):
+    """chimera_adj_record runs the last segment forward (output = the forward steps'
+    bits) and leaves its trajectory for the sweep, which then gives the bits of a sweep
+    that recomputes it; a recording with other dts is not used."""
+    torch = _torch()
+    from chimera_amd.adjoint import AdjointStepper
+    p = problems.field_loop(96, 48, scheme=(2, 2, 1, 1))
+    rng = np.random.default_rng(17)
+    dts = 0.3 * p.dx / 3.0 * rng.uniform(0.6, 1.0, nsteps)
+    x = torch.from_numpy(p.state.copy()).cuda()
+    adj = AdjointStepper(p.nx, p.ny, p.dx, p.gamma, p.bc, p.scheme)
+    ckpts = []
+    for s in range(nsteps):
+        if s % K == 0:
+            ckpts.append(x.clone())
+        x = adj.forward_step(x, dts[s])
+    nseg, s_last = len(ckpts), (len(ckpts) - 1) * K
+    lam0 = torch.from_numpy(rng.standard_normal(p.state.shape)).cuda()
+    ref = adj.sweep(ckpts, dts, K, lam0.clone()).cpu().numpy()
+    out = adj.record(ckpts[-1], dts[s_last:], K, nseg, torch.empty_like(x))
+    assert np.array_equal(out.cpu().numpy(), x.cpu().numpy())
+    assert np.array_equal(adj.sweep(ckpts, dts, K, lam0.clone()).cpu().numpy(), ref)
+    adj.record(ckpts[-1], dts[s_last:] * 1.01, K, nseg, torch.empty_like(x))
+    assert np.array_equal(adj.sweep(ckpts, dts, K, lam0.clone()).cpu().numpy(), ref)
+    adj.close()
