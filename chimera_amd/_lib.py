@@ -142,6 +142,7 @@ def _declare(L: C.CDLL) -> None:
         "chimera_badj_forward": [vp, C.POINTER(vp), c_dp, i, C.POINTER(vp)],
         "chimera_badj_segment_vjp": [vp, C.POINTER(vp), c_dp, i, C.POINTER(vp)],
         "chimera_badj_sweep": [vp, C.POINTER(vp), c_dp, i, i, C.POINTER(vp)],
+        "chimera_badj_record": [vp, C.POINTER(vp), c_dp, i, i, i, C.POINTER(vp)],
         "chimera_badj_set_tiles": [vp, i, i, i],
         "chimera_badj_autotune": [vp, C.POINTER(vp), c_dp, i, C.POINTER(i), C.POINTER(C.c_double)],
     }

@@ -259,6 +259,20 @@ class BatchAdjoint:
                                          d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
                                          int(seg_steps), _ptrs_many(lams)), "chimera_badj_sweep")
 
+    def record(self, states0, dts, seg_steps: int, nseg: int, out=None):
+        """dts.shape[1] (<= seg_steps) steps of every member from states0 into out (new
+        tensors unless given), keeping the trajectories the next sweep of a run of nseg
+        segments checkpointed every seg_steps steps would recompute for its last
+        segment (with states0 as that segment's checkpoint and the same dts)."""
+        d = _dt_table(dts, self.members)
+        out = [torch.empty_like(x) for x in states0] if out is None else out
+        torch.cuda.synchronize()
+        check(self._L.chimera_badj_record(self._h, _ptrs_many(states0),
+                                          d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
+                                          int(seg_steps), int(nseg), _ptrs_many(out)),
+              "chimera_badj_record")
+        return out
+
     def set_tiles(self, shape_a: int = -1, shape_b: int = -1, shape_f: int = -1) -> None:
         """Tile shapes of the two reverse passes and of the forward stages: 0 = 16x16,
         1 = 32x8 (-1 keeps)."""
@@ -293,12 +307,18 @@ class _EvolveBatch(torch.autograd.Function):
     def forward(ctx, badj, dts, K, *U0s):
         states = [u.detach().contiguous().clone() for u in U0s]
         n = dts.shape[1]
+        nseg = -(-n // K)
+        s_last = (nseg - 1) * K
         ckpts = [[s.clone() for s in states]]
-        for s0 in range(0, n, K):
-            k = min(K, n - s0)
-            badj.forward(states, dts[:, s0:s0 + k], out=states)
-            if s0 + k < n:
-                ckpts.append([s.clone() for s in states])
+        for s0 in range(0, s_last, K):
+            badj.forward(states, dts[:, s0:s0 + K], out=states)
+            ckpts.append([s.clone() for s in states])
+        last = np.ascontiguousarray(dts[:, s_last:])
+        if any(ctx.needs_input_grad[3:]):
+            # the backward's sweep finds the last segment's trajectory in place
+            badj.record(ckpts[-1], last, K, nseg, out=states)
+        else:
+            badj.forward(states, last, out=states)
         ctx.ckpts, ctx.badj, ctx.dts, ctx.K = ckpts, badj, dts, K
         return tuple(states)
 

@@ -256,6 +256,13 @@ struct chimera_badj {
     // the whole-run sweep: one member table per segment
     BAdjMember *dseg = nullptr;
     int dseg_cap = 0;              // segments the tables hold
+    // the last segment of a run recorded by chimera_badj_record for the next sweep
+    struct Recorded {
+        bool valid = false;
+        std::vector<const double *> state0;   // members x 6
+        std::vector<double> dts;              // members x nsteps
+        int nsteps = 0, K = 0, nseg = 0;
+    } rec;
 };
 
 namespace {
@@ -323,6 +330,7 @@ int badj_prepare(chimera_badj *b, const double *dt_in, int nsteps, int traj_step
         for (int s = 0; s < nsteps; ++s) h[(size_t)m * b->dt_cap + s] = dt_in[(size_t)m * nsteps + s];
     CH_HIP(hipMemcpy(b->dts, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     if (traj_steps > b->traj_cap) {
+        b->rec.valid = false;   // a recorded segment does not survive a reallocation
         for (auto *v : {&b->traj, &b->uptraj})
             for (auto &p : *v) {
                 (void)hipFree(p);
@@ -492,6 +500,7 @@ extern "C" int chimera_badj_forward(chimera_badj_t *b, const double *const *stat
     CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
     for (long long k = 0; k < (long long)b->members * nsteps; ++k)
         CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
+    b->rec.valid = false;   // slots 1 and 2 of the trajectory are overwritten
     CH_TRY(badj_prepare(b, dt_in, nsteps, 1));   // slots 1 and 2 of one step
     CH_TRY(badj_upload(b, state_in, nullptr, state_out));
     CH_TRY(badj_forward(b, b->dmem, nsteps, true));
@@ -508,11 +517,47 @@ extern "C" int chimera_badj_segment_vjp(chimera_badj_t *b, const double *const *
     CH_REQUIRE(nsteps >= 1, "nsteps must be >= 1");
     for (long long k = 0; k < (long long)b->members * nsteps; ++k)
         CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
+    b->rec.valid = false;
     CH_TRY(badj_prepare(b, dt_in, nsteps, nsteps));
     CH_TRY(badj_upload(b, state0, lam, nullptr));
     CH_TRY(badj_forward(b, b->dmem, nsteps, false));
     CH_TRY(badj_reverse(b, b->dmem, nsteps));
     CH_HIP(hipStreamSynchronize(b->stream));
+    return CHIMERA_OK;
+}
+
+// Forward of the last checkpoint segment of a run for every member (nsteps <= seg_steps
+// steps from state0, members x 6 device pointers, dts dt_in members x nsteps, host),
+// keeping the trajectories chimera_badj_sweep of a run of nseg segments would
+// recompute for that segment; the final states go to state_out (members x 6).
+extern "C" int chimera_badj_record(chimera_badj_t *b, const double *const *state0,
+                                   const double *dt_in, int nsteps, int seg_steps, int nseg,
+                                   double *const *state_out) {
+    CH_REQUIRE(b && state0 && dt_in && state_out, "NULL argument");
+    CH_REQUIRE(nsteps >= 1 && nseg >= 1 && seg_steps >= nsteps,
+               "need 1 <= nsteps <= seg_steps and nseg >= 1 (got %d, %d, %d)", nsteps, seg_steps,
+               nseg);
+    for (long long k = 0; k < (long long)b->members * nsteps; ++k)
+        CH_REQUIRE(dt_in[k] > 0.0, "dt_in[%lld] = %g is not positive", k, dt_in[k]);
+    b->rec.valid = false;
+    const int K = nseg > 1 ? seg_steps : nsteps;
+    CH_TRY(badj_prepare(b, dt_in, nsteps, K));
+    CH_TRY(badj_upload(b, state0, nullptr, nullptr));
+    CH_TRY(badj_forward(b, b->dmem, nsteps, false));
+    for (int m = 0; m < b->members; ++m) {
+        const BAdjMember &h = b->hmem[m];
+        for (int q = 0; q < 6; ++q)
+            CH_HIP(hipMemcpyAsync(state_out[6 * (size_t)m + q],
+                                  b->traj[m] + ((size_t)6 * (3 * nsteps) + q) * h.n,
+                                  h.n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    }
+    CH_HIP(hipStreamSynchronize(b->stream));
+    b->rec.valid = true;
+    b->rec.state0.assign(state0, state0 + 6 * (size_t)b->members);
+    b->rec.dts.assign(dt_in, dt_in + (size_t)b->members * nsteps);
+    b->rec.nsteps = nsteps;
+    b->rec.K = K;
+    b->rec.nseg = nseg;
     return CHIMERA_OK;
 }
 
@@ -534,8 +579,23 @@ extern "C" int chimera_badj_sweep(chimera_badj_t *b, const double *const *states
     const int nseg = (nsteps + seg_steps - 1) / seg_steps;
     for (long long k = 0; k < 6LL * nseg * b->members; ++k)
         CH_REQUIRE(states[k], "states[%lld] is NULL", k);
-    CH_TRY(badj_prepare(b, dt_in, nsteps, std::min(seg_steps, nsteps)));
+    const int K = std::min(seg_steps, nsteps);
+    CH_TRY(badj_prepare(b, dt_in, nsteps, K));
     CH_TRY(ensure_sweep(b, nseg));
+    // the last segment recorded by chimera_badj_record for exactly this run (checked
+    // after the allocations, which drop a recording when they reallocate)?
+    const int last_len = nsteps - (nseg - 1) * seg_steps;
+    bool have_last = b->rec.valid && b->rec.nseg == nseg && b->rec.K == K &&
+                     b->rec.nsteps == last_len;
+    for (int m = 0; have_last && m < b->members; ++m) {
+        for (int q = 0; q < 6; ++q)
+            have_last = have_last && b->rec.state0[6 * (size_t)m + q] ==
+                                         states[6 * ((size_t)(nseg - 1) * b->members + m) + q];
+        for (int s = 0; s < last_len; ++s)
+            have_last = have_last && b->rec.dts[(size_t)m * last_len + s] ==
+                                         dt_in[(size_t)m * nsteps + (size_t)(nseg - 1) * seg_steps + s];
+    }
+    b->rec.valid = false;
     // one member table per segment: its checkpoint and dt offset
     std::vector<BAdjMember> tabs((size_t)nseg * b->members);
     for (int k = 0; k < nseg; ++k)
@@ -556,7 +616,7 @@ extern "C" int chimera_badj_sweep(chimera_badj_t *b, const double *const *states
     for (int k = nseg - 1; k >= 0; --k) {
         const BAdjMember *tab = b->dseg + (size_t)k * b->members;
         const int len = std::min(seg_steps, nsteps - k * seg_steps);
-        CH_TRY(badj_forward(b, tab, len, false));
+        if (!(have_last && k == nseg - 1)) CH_TRY(badj_forward(b, tab, len, false));
         CH_TRY(badj_reverse(b, tab, len));
     }
     CH_HIP(hipStreamSynchronize(b->stream));
@@ -588,6 +648,7 @@ extern "C" int chimera_badj_autotune(chimera_badj_t *b, const double *const *sta
     CH_REQUIRE(reps >= 1, "reps must be >= 1");
     for (int m = 0; m < b->members; ++m)
         CH_REQUIRE(dt_in[m] > 0.0, "dt_in[%d] = %g is not positive", m, dt_in[m]);
+    b->rec.valid = false;
     CH_TRY(badj_prepare(b, dt_in, 1, 1));
     // scratch: 6 arrays per member, the pure forward's output and then lam
     size_t total = 0;

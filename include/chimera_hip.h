@@ -339,6 +339,14 @@ int chimera_badj_segment_vjp(chimera_badj_t *badj, const double *const *state0,
  * counterpart in the reference (it has no derivatives). */
 int chimera_badj_sweep(chimera_badj_t *badj, const double *const *states, const double *dt_in,
                        int nsteps, int seg_steps, double *const *lam);
+/* Forward of the last checkpoint segment of a run for every member (nsteps <= seg_steps
+ * steps from state0, members x 6 device pointers; dt_in members x nsteps, host), keeping
+ * the trajectories the next chimera_badj_sweep of a run of nseg segments would
+ * recompute for it; the final states go to state_out (members x 6).  That sweep skips
+ * the recompute when its last segment has the same state0 pointers and dts; any other
+ * trajectory use drops the recording.  No counterpart in the reference. */
+int chimera_badj_record(chimera_badj_t *badj, const double *const *state0, const double *dt_in,
+                        int nsteps, int seg_steps, int nseg, double *const *state_out);
 /* Tile shapes of the two reverse passes (default 1) and of the forward stages
  * (default 0): 0 = 16x16, 1 = 32x8 (-1 keeps).  Every shape gives the same bits. */
 int chimera_badj_set_tiles(chimera_badj_t *badj, int shape_a, int shape_b, int shape_f);

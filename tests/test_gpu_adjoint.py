@@ -445,6 +445,19 @@ def test_batched_sweep_matches_segment_loop(gpu_lib, nsteps, K):
         badj.sweep(ckpts, dts, K, got)
         for m in range(len(probs)):
             assert np.array_equal(got[m].cpu().numpy(), ref[m].cpu().numpy()), m
+    # a recorded last segment (output = the forward's bits) is reused; one recorded
+    # with other dts is not
+    nseg, s_last = len(ckpts), (len(ckpts) - 1) * K
+    last = np.ascontiguousarray(dts[:, s_last:])
+    for scale in (1.0, 1.01):
+        out = badj.record(ckpts[-1], last * scale, K, nseg)
+        if scale == 1.0:
+            for m in range(len(probs)):
+                assert np.array_equal(out[m].cpu().numpy(), states[m].cpu().numpy()), m
+        got = [l.clone() for l in lam0]
+        badj.sweep(ckpts, dts, K, got)
+        for m in range(len(probs)):
+            assert np.array_equal(got[m].cpu().numpy(), ref[m].cpu().numpy()), (scale, m)
     badj.close()
 
 
