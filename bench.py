@@ -55,6 +55,8 @@ def parse_args():
     ap.add_argument("--dgrf-members", type=int, default=64,
                     help="members of the differentiable GRF section (0: skip)")
     ap.add_argument("--dgrf-steps", type=int, default=300)
+    ap.add_argument("--strip-steps", type=int, default=200,
+                    help="steps of the one-rank strip of an 8-way rotor split (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of 2^26 doubles (512 MiB each way) 3 "
                          "times, the reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -288,6 +290,45 @@ def adjoint_field_loop(args, torch) -> dict:
             "cost_ratio": t_bwd / t_fwd, "fd_check": fd}
 
 
+def shard_strip(args, torch, full_stage_ms: float) -> dict:
+    """One GPU's share of the rotor n x n split into 8 strips: its n/8 owned rows plus
+    the depth-3 halo rows on each side, cut from the centre of the rotor state, through
+    the same resident path (outflow BCs stand in for the received halo).  Reports the
+    stage kernel per owned cell against the full grid's and the halo bytes a stage
+    would exchange over xGMI."""
+    from chimera_amd import problems
+    from chimera_amd.solver import Grid, DT_CFL
+    n, halo = args.n, 3
+    owned = n // 8
+    rows = owned + 2 * halo
+    full = problems.rotor(n, scheme=(2, 1, 1, 1))
+    j0 = n // 2 - rows // 2
+    prob = problems.Problem("rotor_strip", full.state[:, j0:j0 + rows, :].copy(), full.dx,
+                            full.gamma, full.bc, full.scheme)
+    g = Grid.from_problem(prob)
+    g.steps(2, dt_mode=DT_CFL)
+    g.autotune(reps=3)
+    g.set_timing(True)
+    g.sync()
+    t0 = time.perf_counter()
+    g.steps(args.strip_steps, dt_mode=DT_CFL)
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    stage_ms, launches = g.stage_kernel_ms()
+    g.close()
+    halo_bytes = 2 * n * halo * 6 * 8   # both sides, 6 fields, FP64
+    full_scaled = full_stage_ms * (n * rows) / (n * n)
+    return {"workload": f"rotor {n}x{n} split into 8 strips: one rank's {n}x{owned} owned "
+                        f"cells + {halo} halo rows per side ({n}x{rows}), HLLE, Van Leer, "
+                        f"{args.strip_steps} steps",
+            "owned_cell_updates_per_s": n * owned * args.strip_steps / elapsed,
+            "stage_ms": stage_ms, "launches": launches,
+            "full_grid_stage_ms_scaled": full_scaled,
+            "stage_efficiency_vs_full": full_scaled / stage_ms if stage_ms > 0 else None,
+            "halo_bytes_per_stage": halo_bytes,
+            "halo_us_per_stage_at_153GBps_per_side": halo_bytes / 2 / 153e9 * 1e6}
+
+
 def differentiable_grf(args, torch) -> dict:
     """Differentiable GRF ensemble: members N in {384, 448, 512} (seed 20261203),
     HLLD, MC, dt frozen from a CFL run of each member; per-member loss =
@@ -430,6 +471,7 @@ def main():
     finite = bool(torch.isfinite(torch.from_numpy(st)).all())
     g.close()
 
+    strip = shard_strip(args, torch, stage_ms) if args.strip_steps > 0 and stage_ms > 0 else None
     batched = batched_grf(args, torch) if args.members > 0 else None
     adjoint = adjoint_field_loop(args, torch) if args.adj_steps > 0 else None
     dgrf = differentiable_grf(args, torch) if args.dgrf_members > 0 else None
@@ -467,6 +509,7 @@ def main():
         "batched_grf": batched,
         "adjoint_field_loop": adjoint,
         "differentiable_grf": dgrf,
+        "shard_strip": strip,
     }
     print(json.dumps(out))
 
