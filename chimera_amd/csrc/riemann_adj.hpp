@@ -338,9 +338,29 @@ __device__ __forceinline__ void rusanov_vjp(const FaceState &L, const FaceState 
 struct HlldSide {
     double r, u, w, pT, B, C;   // rho, u_n, u_t, total pressure, B_n, B_t
     SideRev e;                  // halfB2, thermal pressure (floor), energy
-    double vdotB, B2, ca, cs2, b2, tmp, s1, cf;
+    double cf;
+    __device__ __forceinline__ double B2() const { return B * B; }
+    __device__ __forceinline__ double vdotB() const { return u * B + w * C; }
+};
+
+// the magnetosonic intermediates of c_f, recomputed with the forward's expressions
+struct HlldCf {
+    double ca, cs2, b2, tmp, s1;
     bool disc_floor;
 };
+
+__device__ __forceinline__ HlldCf hlld_cf_parts(const HlldSide &h, double g) {
+    HlldCf c;
+    const double B2 = h.B2();
+    c.ca = B2 / h.r;
+    c.cs2 = g * h.e.pth / h.r;
+    c.b2 = (B2 + h.C * h.C) / h.r;
+    c.tmp = c.cs2 + c.b2;
+    const double d = c.tmp * c.tmp - 4.0 * c.cs2 * c.ca;
+    c.disc_floor = 0.0 > d;
+    c.s1 = sqrt(c.disc_floor ? 0.0 : d);
+    return c;
+}
 
 __device__ __forceinline__ HlldSide hlld_side(const FaceState &s, double g) {
     HlldSide h;
@@ -351,16 +371,8 @@ __device__ __forceinline__ HlldSide hlld_side(const FaceState &s, double g) {
     h.B = s.Bn;
     h.C = s.Bt;
     h.e = side_energy(s, g);
-    h.vdotB = h.u * h.B + h.w * h.C;
-    h.B2 = h.B * h.B;
-    h.ca = h.B2 / h.r;
-    h.cs2 = g * h.e.pth / h.r;
-    h.b2 = (h.B2 + h.C * h.C) / h.r;
-    h.tmp = h.cs2 + h.b2;
-    const double d = h.tmp * h.tmp - 4.0 * h.cs2 * h.ca;
-    h.disc_floor = 0.0 > d;
-    h.s1 = sqrt(h.disc_floor ? 0.0 : d);
-    h.cf = sqrt(0.5 * (h.tmp + h.s1));
+    const HlldCf c = hlld_cf_parts(h, g);
+    h.cf = sqrt(0.5 * (c.tmp + c.s1));
     return h;
 }
 
@@ -369,49 +381,55 @@ __device__ __forceinline__ void hlld_phys(const HlldSide &h, double F[5]) {
     F[0] = h.r * h.u;
     F[1] = h.r * (h.u * h.u) + h.pT - h.B * h.B;
     F[2] = h.r * h.u * h.w - h.B * h.C;
-    F[3] = (h.e.en + h.pT) * h.u - h.B * h.vdotB;
+    F[3] = (h.e.en + h.pT) * h.u - h.B * h.vdotB();
     F[4] = h.C * h.u - h.B * h.w;
 }
 
 // adjoints gathered per side before they reach its six inputs
 struct HlldAdj {
     double l[6];
-    double len, lpth, lhalf, lB2, lvdotB, lF[5];
+    double len, lpth, lhalf, lB2, lvdotB;
+    double cF;   // lam of the side's physical flux = cF * lamF
 };
 
 __device__ __forceinline__ void hlld_adj_zero(HlldAdj &a) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) a.l[q] = 0.0;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) a.lF[q] = 0.0;
+    a.cF = 0.0;
     a.len = a.lpth = a.lhalf = a.lB2 = a.lvdotB = 0.0;
 }
 
 // c_f = sqrt(0.5 (tmp + sqrt(max(tmp^2 - 4 cs2 ca, 0)))) and its inputs
 __device__ __forceinline__ void hlld_cf_rev(const HlldSide &h, double g, double lcf, HlldAdj &a) {
+    const HlldCf c = hlld_cf_parts(h, g);
     const double lx = h.cf > 0.0 ? lcf * (0.5 / h.cf) : 0.0;
     double ltmp = 0.5 * lx;
     const double ls1 = 0.5 * lx;
-    const double ld = (!h.disc_floor && h.s1 > 0.0) ? ls1 * (0.5 / h.s1) : 0.0;
-    ltmp += ld * (2.0 * h.tmp);
-    double lcs2 = -ld * (4.0 * h.ca);
-    const double lca = -ld * (4.0 * h.cs2);
+    const double ld = (!c.disc_floor && c.s1 > 0.0) ? ls1 * (0.5 / c.s1) : 0.0;
+    ltmp += ld * (2.0 * c.tmp);
+    double lcs2 = -ld * (4.0 * c.ca);
+    const double lca = -ld * (4.0 * c.cs2);
     lcs2 += ltmp;
     const double lb2 = ltmp;
     const double ir = 1.0 / h.r;   // one reciprocal for the seven quotients below
     a.lB2 += lb2 * ir;
     a.l[5] += lb2 * (2.0 * h.C) * ir;
-    a.l[0] -= lb2 * h.b2 * ir;
+    a.l[0] -= lb2 * c.b2 * ir;
     a.lpth += lcs2 * g * ir;
-    a.l[0] -= lcs2 * h.cs2 * ir;
+    a.l[0] -= lcs2 * c.cs2 * ir;
     a.lB2 += lca * ir;
-    a.l[0] -= lca * h.ca * ir;
+    a.l[0] -= lca * c.ca * ir;
 }
 
 // phys flux, v.B, B^2 and energy of one side -> its inputs (written into lam)
 __device__ __forceinline__ void hlld_side_finish(const FaceState &s, const HlldSide &h, double g,
-                                                 HlldAdj &a, double lam[6]) {
-    phys_flux_rev(s, h.e.en, a.lF, a.l, a.len);
+                                                 const double lamF[5], HlldAdj &a, double lam[6]) {
+    if (a.cF != 0.0) {
+        double lF[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lF[c] = a.cF * lamF[c];
+        phys_flux_rev(s, h.e.en, lF, a.l, a.len);
+    }
     a.l[1] += a.lvdotB * h.B;
     a.l[4] += a.lvdotB * h.u;
     a.l[2] += a.lvdotB * h.C;
@@ -434,8 +452,8 @@ __device__ __forceinline__ HlldStar hlld_star(const HlldSide &h, double S, doubl
     st.dS = S - SM;
     st.rs = st.a / st.dS;
     st.k = st.a * st.dS;
-    st.den = st.k - h.B2;
-    st.cond = fabs(st.den) > 1.0e-3 * fmax(fmax(fabs(st.k), h.B2), 1.0e-30);
+    st.den = st.k - h.B2();
+    st.cond = fabs(st.den) > 1.0e-3 * fmax(fmax(fabs(st.k), h.B2()), 1.0e-30);
     st.vs = h.w;
     st.Bs = h.C;
     st.t = S - h.u;
@@ -443,10 +461,10 @@ __device__ __forceinline__ HlldStar hlld_star(const HlldSide &h, double S, doubl
     if (st.cond) {
         st.nv = h.B * h.C * (SM - h.u);
         st.vs = h.w - st.nv / st.den;
-        st.Bs = h.C * (h.r * (st.t * st.t) - h.B2) / st.den;
+        st.Bs = h.C * (h.r * (st.t * st.t) - h.B2()) / st.den;
     }
     st.vdotBs = SM * h.B + st.vs * st.Bs;
-    st.ens = (h.e.en * (S - h.u) - h.pT * h.u + pTs * SM + h.B * (h.vdotB - st.vdotBs)) / st.dS;
+    st.ens = (h.e.en * (S - h.u) - h.pT * h.u + pTs * SM + h.B * (h.vdotB() - st.vdotBs)) / st.dS;
     st.sq_floor = RHO_FLOOR > st.rs;
     st.sq = sqrt(st.sq_floor ? RHO_FLOOR : st.rs);
     return st;
@@ -472,7 +490,7 @@ __device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar 
     a.l[1] -= lnumE * h.pT;
     lpTs += lnumE * SM;
     lSM += lnumE * pTs;
-    a.l[4] += lnumE * (h.vdotB - st.vdotBs);
+    a.l[4] += lnumE * (h.vdotB() - st.vdotBs);
     a.lvdotB += lnumE * h.B;
     lvdotBs -= lnumE * h.B;
     // vdotBs = SM B + vs Bs
@@ -487,7 +505,7 @@ __device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar 
         // Bs = C (r t^2 - B^2) / den
         const double lnb = lBs * iden;
         lden -= lBs * st.Bs * iden;
-        a.l[5] += lnb * (h.r * (st.t * st.t) - h.B2);
+        a.l[5] += lnb * (h.r * (st.t * st.t) - h.B2());
         const double lin = lnb * h.C;
         a.l[0] += lin * (st.t * st.t);
         const double lt = lin * h.r * (2.0 * st.t);
@@ -534,8 +552,6 @@ __device__ __forceinline__ void hll_avg_rev(const HlldSide &hL, const HlldSide &
         const double num = SR * FL[c] - SL * FR[c] + sLR * dU[c];
         const double lc = lamF[c];
         linv += lc * num;
-        aL.lF[c] += lc * SR * inv;
-        aR.lF[c] -= lc * SL * inv;
         const double ld = lc * sLR * inv;
         lSR += lc * inv * (FL[c] + SL * dU[c]);
         lSL += lc * inv * (SR * dU[c] - FR[c]);
@@ -560,6 +576,8 @@ __device__ __forceinline__ void hll_avg_rev(const HlldSide &hL, const HlldSide &
             aL.l[5] -= ld;
         }
     }
+    aL.cF += SR * inv;
+    aR.cF -= SL * inv;
     const double lspan = -linv * inv * inv;
     lSR += lspan;
     lSL -= lspan;
@@ -569,9 +587,6 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
                                          const double lamF[5], double lamL[6], double lamR[6]) {
     const double EPS_FAN = 1.0e-6, cf_max = 50.0;
     const HlldSide hL = hlld_side(L, g), hR = hlld_side(R, g);
-    double FL[5], FR[5];
-    hlld_phys(hL, FL);
-    hlld_phys(hR, FR);
     HlldAdj aL, aR;
     hlld_adj_zero(aL);
     hlld_adj_zero(aR);
@@ -589,16 +604,13 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
     bool speeds = true;   // SL / SR (hence c_f) enter the flux
     if (cfL >= cf_max || cfR >= cf_max) {
         HlldAdj &a = (cfL <= cfR) ? aL : aR;
-#pragma unroll
-        for (int c = 0; c < 5; ++c) a.lF[c] += lamF[c];
+        a.cF += 1.0;
         speeds = false;
     } else if (SL >= 0.0) {
-#pragma unroll
-        for (int c = 0; c < 5; ++c) aL.lF[c] += lamF[c];
+        aL.cF += 1.0;
         speeds = false;
     } else if (SR <= 0.0) {
-#pragma unroll
-        for (int c = 0; c < 5; ++c) aR.lF[c] += lamF[c];
+        aR.cF += 1.0;
         speeds = false;
     } else {
         const double aLv = hL.r * (SL - hL.u), aRv = hR.r * (SR - hR.u);
@@ -640,7 +652,8 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
             const HlldStar &sX = left ? sL : sR;
             const double S = left ? SL : SR, Ss = left ? SLs : SRs;
             const double enXss = left ? enLss : enRss;
-            const double *FX = left ? FL : FR;
+            double FX[5];
+            hlld_phys(left ? hL : hR, FX);
             const bool star2 = left ? !(SLs >= 0.0) : !(SRs <= 0.0);
             double f[5];
             f[0] = FX[0] + S * (sX.rs - hX.r);
@@ -654,6 +667,9 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
                 f[4] = f[4] + Ss * (Bss - sX.Bs);
             }
             const double Smax = fmax(fabs(SL), fabs(SR));
+            double FL[5], FR[5];
+            hlld_phys(hL, FL);
+            hlld_phys(hR, FR);
             const bool ok =
                 finite_in(f[0], fmax(fabs(FL[0]), fabs(FR[0])) + Smax * (hL.r + hR.r)) &&
                 finite_in(f[1], fmax(fabs(FL[1]), fabs(FR[1])) +
@@ -670,8 +686,7 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
                 const double l0 = lamF[0], l1 = lamF[1], l2 = lamF[2], l3 = lamF[3], l4 = lamF[4];
                 double lS = 0.0, lSM = 0.0, lrs = 0.0, lvs = 0.0, lBs = 0.0, lens = 0.0;
                 double lSs = 0.0, lvss = 0.0, lBss = 0.0, lenss = 0.0;
-#pragma unroll
-                for (int c = 0; c < 5; ++c) aX.lF[c] += lamF[c];
+                aX.cF += 1.0;
                 lS += l0 * (sX.rs - hX.r);
                 lrs += l0 * S;
                 aX.l[0] -= l0 * S;
@@ -810,7 +825,12 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
                 aR.l[1] -= laR * hR.r;
             }
         }
-        if (hll) hll_avg_rev(hL, hR, FL, FR, SL, SR, lamF, aL, aR, lSL, lSR);
+        if (hll) {
+            double FL[5], FR[5];
+            hlld_phys(hL, FL);
+            hlld_phys(hR, FR);
+            hll_avg_rev(hL, hR, FL, FR, SL, SR, lamF, aL, aR, lSL, lSR);
+        }
     }
     if (speeds) {
         // SL = min(uL - cfL, uR - cfR), SR = max(uL + cfL, uR + cfR); no cap on this path
@@ -832,8 +852,8 @@ __device__ __forceinline__ void hlld_vjp(const FaceState &L, const FaceState &R,
         if (!capL) hlld_cf_rev(hL, g, lcfL, aL);
         if (!capR) hlld_cf_rev(hR, g, lcfR, aR);
     }
-    hlld_side_finish(L, hL, g, aL, lamL);
-    hlld_side_finish(R, hR, g, aR, lamR);
+    hlld_side_finish(L, hL, g, lamF, aL, lamL);
+    hlld_side_finish(R, hR, g, lamF, aR, lamR);
 }
 
 // Face VJP of the solver SOLVER, reverse mode (by hand) for all three solvers; the
