@@ -440,31 +440,42 @@ __device__ __forceinline__ void hlld_side_finish(const FaceState &s, const HlldS
     for (int q = 0; q < 6; ++q) lam[q] = a.l[q];
 }
 
-// one side's star state (Miyoshi-Kusano): forward values
+// one side's star state (Miyoshi-Kusano): the forward values the rest of the face
+// needs; the cheap intermediates are rebuilt by hlld_star_parts
 struct HlldStar {
-    double a, dS, rs, k, den, t, nv, vs, Bs, vdotBs, ens, sq;
-    bool cond, sq_floor;
+    double rs, vs, Bs, vdotBs, ens, sq;
+    bool sq_floor;
 };
 
+struct HlldStarParts {
+    double a, dS, k, den, t, nv;
+    bool cond;
+};
+
+__device__ __forceinline__ HlldStarParts hlld_star_parts(const HlldSide &h, double S, double SM) {
+    HlldStarParts p;
+    p.a = h.r * (S - h.u);
+    p.dS = S - SM;
+    p.k = p.a * p.dS;
+    p.den = p.k - h.B2();
+    p.cond = fabs(p.den) > 1.0e-3 * fmax(fmax(fabs(p.k), h.B2()), 1.0e-30);
+    p.t = S - h.u;
+    p.nv = p.cond ? h.B * h.C * (SM - h.u) : 0.0;
+    return p;
+}
+
 __device__ __forceinline__ HlldStar hlld_star(const HlldSide &h, double S, double SM, double pTs) {
+    const HlldStarParts p = hlld_star_parts(h, S, SM);
     HlldStar st;
-    st.a = h.r * (S - h.u);
-    st.dS = S - SM;
-    st.rs = st.a / st.dS;
-    st.k = st.a * st.dS;
-    st.den = st.k - h.B2();
-    st.cond = fabs(st.den) > 1.0e-3 * fmax(fmax(fabs(st.k), h.B2()), 1.0e-30);
+    st.rs = p.a / p.dS;
     st.vs = h.w;
     st.Bs = h.C;
-    st.t = S - h.u;
-    st.nv = 0.0;
-    if (st.cond) {
-        st.nv = h.B * h.C * (SM - h.u);
-        st.vs = h.w - st.nv / st.den;
-        st.Bs = h.C * (h.r * (st.t * st.t) - h.B2()) / st.den;
+    if (p.cond) {
+        st.vs = h.w - p.nv / p.den;
+        st.Bs = h.C * (h.r * (p.t * p.t) - h.B2()) / p.den;
     }
     st.vdotBs = SM * h.B + st.vs * st.Bs;
-    st.ens = (h.e.en * (S - h.u) - h.pT * h.u + pTs * SM + h.B * (h.vdotB() - st.vdotBs)) / st.dS;
+    st.ens = (h.e.en * (S - h.u) - h.pT * h.u + pTs * SM + h.B * (h.vdotB() - st.vdotBs)) / p.dS;
     st.sq_floor = RHO_FLOOR > st.rs;
     st.sq = sqrt(st.sq_floor ? RHO_FLOOR : st.rs);
     return st;
@@ -477,9 +488,10 @@ __device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar 
                                               double lBs, double lvdotBs, double lens, double lsq,
                                               HlldAdj &a, double &lS, double &lSM, double &lpTs,
                                               double &la) {
+    const HlldStarParts p = hlld_star_parts(h, S, SM);
     if (!st.sq_floor && st.sq > 0.0) lrs += lsq * (0.5 / st.sq);
     double ldS = 0.0;
-    const double idS = 1.0 / st.dS;   // reverse only: one reciprocal per denominator
+    const double idS = 1.0 / p.dS;   // reverse only: one reciprocal per denominator
     // ens = numE / dS
     const double lnumE = lens * idS;
     ldS -= lens * st.ens * idS;
@@ -499,23 +511,23 @@ __device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar 
     lvs += lvdotBs * st.Bs;
     lBs += lvdotBs * st.vs;
     double lk = 0.0;
-    if (st.cond) {
+    if (p.cond) {
         double lden = 0.0;
-        const double iden = 1.0 / st.den;
+        const double iden = 1.0 / p.den;
         // Bs = C (r t^2 - B^2) / den
         const double lnb = lBs * iden;
         lden -= lBs * st.Bs * iden;
-        a.l[5] += lnb * (h.r * (st.t * st.t) - h.B2());
+        a.l[5] += lnb * (h.r * (p.t * p.t) - h.B2());
         const double lin = lnb * h.C;
-        a.l[0] += lin * (st.t * st.t);
-        const double lt = lin * h.r * (2.0 * st.t);
+        a.l[0] += lin * (p.t * p.t);
+        const double lt = lin * h.r * (2.0 * p.t);
         a.lB2 -= lin;
         lS += lt;
         a.l[1] -= lt;
         // vs = w - nv / den, nv = B C (SM - u)
         a.l[2] += lvs;
         const double lnv = -lvs * iden;
-        lden += lvs * (st.nv * iden) * iden;
+        lden += lvs * (p.nv * iden) * iden;
         a.l[4] += lnv * h.C * (SM - h.u);
         a.l[5] += lnv * h.B * (SM - h.u);
         lSM += lnv * (h.B * h.C);
@@ -528,8 +540,8 @@ __device__ __forceinline__ void hlld_star_rev(const HlldSide &h, const HlldStar 
         a.l[5] += lBs;
     }
     // k = a dS, rs = a / dS, dS = S - SM
-    la += lk * st.dS;
-    ldS += lk * st.a;
+    la += lk * p.dS;
+    ldS += lk * p.a;
     la += lrs * idS;
     ldS -= lrs * st.rs * idS;
     lS += ldS;
