@@ -427,7 +427,7 @@ def main():
 
     import torch
     from chimera_amd import problems
-    from chimera_amd.solver import Grid, DT_CFL
+    from chimera_amd.solver import Grid, DT_CFL, TILE_SHAPES
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -496,7 +496,7 @@ def main():
                    "nx": n, "ny": n, "path": "plain resident (fused stage kernel)",
                    "tile": [tile[1], tile[2]],
                    "tile_autotune_ms": {f"{a}x{b}": round(t, 4) for (a, b), t in
-                                        zip(((32, 8), (16, 16), (64, 4), (32, 32)), tile_ms)}},
+                                        zip(TILE_SHAPES, tile_ms)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic_from_profiles(n),

@@ -61,20 +61,21 @@ using StageFn = void (*)(GridView, int, int, int, int, int);
 
 #define CH_SHAPES(S, L)                                                                 \
     {stage_kernel<S, L, 32, 8>, stage_kernel<S, L, 16, 16>, stage_kernel<S, L, 64, 4>,     \
-     stage_kernel<S, L, 32, 32>}
-#define CH_SHAPES_256(S, L)                                                             \
-    {stage_kernel<S, L, 32, 8>, stage_kernel<S, L, 16, 16>, stage_kernel<S, L, 64, 4>, nullptr}
+     stage_kernel<S, L, 32, 32>, stage_kernel<S, L, 32, 16>}
+#define CH_SHAPES_HLLD(S, L)                                                            \
+    {stage_kernel<S, L, 32, 8>, stage_kernel<S, L, 16, 16>, stage_kernel<S, L, 64, 4>,     \
+     nullptr, stage_kernel<S, L, 32, 16>}
 
 // nullptr: shape not available for that solver
 StageFn stage_fn(int solver, int limiter, int shape) {
     static const StageFn table[3][2][RES_NUM_SHAPES] = {
         {CH_SHAPES(RUSANOV, LIM_VAN_LEER), CH_SHAPES(RUSANOV, LIM_MC)},
         {CH_SHAPES(HLLE, LIM_VAN_LEER), CH_SHAPES(HLLE, LIM_MC)},
-        {CH_SHAPES_256(HLLD, LIM_VAN_LEER), CH_SHAPES_256(HLLD, LIM_MC)}};
+        {CH_SHAPES_HLLD(HLLD, LIM_VAN_LEER), CH_SHAPES_HLLD(HLLD, LIM_MC)}};
     return table[solver - 1][limiter - 1][shape];
 }
 #undef CH_SHAPES
-#undef CH_SHAPES_256
+#undef CH_SHAPES_HLLD
 
 __device__ __forceinline__ double wave_max(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));

@@ -22,9 +22,12 @@ constexpr int STAGE_THREADS = 256;
 // The single-grid driver also has a 32 x 32 tile run by 1024 threads (Rusanov and
 // HLLE; HLLD needs more than the 128 VGPRs a 1024-thread block leaves each lane): per
 // cell 1.27 primitive conversions and 2.19 Riemann faces instead of 1.69 and 2.48 for
-// 32 x 8, in 148 KiB of LDS, one block (16 waves) per CU.
-constexpr int RES_NUM_SHAPES = 4;
-constexpr TileDims RES_TILE_SHAPES[RES_NUM_SHAPES] = {{32, 8}, {16, 16}, {64, 4}, {32, 32}};
+// 32 x 8, in 144 KiB of LDS, one block (16 waves) per CU; and a 32 x 16 tile run by 512
+// threads (1.41 conversions, 2.29 faces) whose 76 KiB of LDS let two blocks share a
+// CU, so that one block's barrier waits overlap the other's work.
+constexpr int RES_NUM_SHAPES = 5;
+constexpr TileDims RES_TILE_SHAPES[RES_NUM_SHAPES] = {
+    {32, 8}, {16, 16}, {64, 4}, {32, 32}, {32, 16}};
 
 // Per-grid step bookkeeping kept on the device so whole steps can be captured.
 struct StepState {

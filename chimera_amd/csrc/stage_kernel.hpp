@@ -50,15 +50,21 @@ struct StageStore {
     double *U3[4];       // after floors and the outflow correction, before the blend
 };
 
+// LDS layout of a tile.  The transverse flux (By on x faces, Bx on y faces) is kept on
+// every face the corner EMFs read, one row / column beyond the tile on each side; the
+// four conserved fluxes only on the faces of the tile's own rows / columns, which is
+// all the flux difference reads.
 template <int TX, int TY>
 struct TileShape {
     static constexpr int PW = TX + 4, PH = TY + 4;     // primitives with a 2-cell halo
     static constexpr int FXW = TX + 1, FXH = TY + 2;   // x faces i in [-1,TX-1], j in [-1,TY]
     static constexpr int FYW = TX + 2, FYH = TY + 1;   // y faces i in [-1,TX],   j in [-1,TY-1]
     static constexpr int P_SZ = PW * PH;
-    static constexpr int FX_SZ = FXW * FXH;
-    static constexpr int FY_SZ = FYW * FYH;
-    static constexpr int LDS_DOUBLES = 6 * P_SZ + 5 * FX_SZ + 5 * FY_SZ;
+    static constexpr int FX_SZ = FXW * FXH;            // transverse flux, all x faces
+    static constexpr int FY_SZ = FYW * FYH;            // transverse flux, all y faces
+    static constexpr int FX4_SZ = FXW * TY;            // conserved fluxes, x faces j in [0,TY-1]
+    static constexpr int FY4_SZ = TX * FYH;            // conserved fluxes, y faces i in [0,TX-1]
+    static constexpr int LDS_DOUBLES = 6 * P_SZ + 4 * FX4_SZ + FX_SZ + 4 * FY4_SZ + FY_SZ;
 };
 
 // Global index of a halo position: periodic wrap or zero-gradient clamp.  Halo
@@ -203,8 +209,10 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
     const int nx = g.nx, ny = g.ny, px = g.px, py = g.py;
     const int x0 = tile_x * TX, y0 = tile_y * TY;
     double *sP = lds;                       // [6][PH][PW]
-    double *sFX = lds + 6 * S::P_SZ;        // [5][FXH][FXW]
-    double *sFY = sFX + 5 * S::FX_SZ;       // [5][FYH][FYW]
+    double *sFX = lds + 6 * S::P_SZ;        // [4][TY][FXW]   conserved fluxes, x faces
+    double *sFY = sFX + 4 * S::FX4_SZ;      // [4][FYH][TX]   conserved fluxes, y faces
+    double *sBX = sFY + 4 * S::FY4_SZ;      // [FXH][FXW]     By flux, x faces
+    double *sBY = sBX + S::FX_SZ;           // [FYH][FYW]     Bx flux, y faces
     const double dx = g.dx, vol = g.vol, gam = g.gamma;
     const double inv_dx = 1.0 / dx;
     const double inv_2dx = 1.0 / (2.0 * dx);
@@ -313,14 +321,18 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             FaceState bR{R[F_RHO], dir ? R[F_VY] : R[F_VX], dir ? R[F_VX] : R[F_VY], R[F_P],
                          dir ? R[F_BY] : R[F_BX], dir ? R[F_BX] : R[F_BY]};
             Flux5 f = riemann<SOLVER>(a, bR, gam);
-            double *sF = dir ? sFY : sFX;
-            const int fsz = dir ? S::FY_SZ : S::FX_SZ;
-            const int idx = dir ? fy * S::FYW + fx : fy * S::FXW + fx;
-            sF[0 * fsz + idx] = f.mass;
-            sF[1 * fsz + idx] = dir ? f.mom_t : f.mom_n;   // Momx
-            sF[2 * fsz + idx] = dir ? f.mom_n : f.mom_t;   // Momy
-            sF[3 * fsz + idx] = f.energy;
-            sF[4 * fsz + idx] = f.bt;                      // By flux (x faces) / Bx flux (y faces)
+            // transverse flux: By flux (x faces) / Bx flux (y faces), every face
+            (dir ? sBY : sBX)[dir ? fy * S::FYW + fx : fy * S::FXW + fx] = f.bt;
+            // conserved fluxes: faces of the tile's own rows (x) / columns (y)
+            if (dir ? (fx >= 1 && fx <= TX) : (fy >= 1 && fy <= TY)) {
+                double *sF = dir ? sFY : sFX;
+                const int fsz = dir ? S::FY4_SZ : S::FX4_SZ;
+                const int idx = dir ? fy * TX + (fx - 1) : (fy - 1) * S::FXW + fx;
+                sF[0 * fsz + idx] = f.mass;
+                sF[1 * fsz + idx] = dir ? f.mom_t : f.mom_n;   // Momx
+                sF[2 * fsz + idx] = dir ? f.mom_n : f.mom_t;   // Momy
+                sF[3 * fsz + idx] = f.energy;
+            }
             if (want_mask && owned && bits)
                 atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
         }
@@ -335,21 +347,21 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         size_t k = (size_t)gi + (size_t)nx * gj;
         const double dt = *g.dt;
         const double dtdx = dt * dx;
-        // x faces: FX(i, j) at [j+1][i+1]; y faces: FY(i, j) at [j+1][i+1]
-        const int xr = (cj + 1) * S::FXW + (ci + 1);
-        const int yr = (cj + 1) * S::FYW + (ci + 1);
+        // conserved fluxes: x faces FX(i, j) at [j][i+1], y faces FY(i, j) at [j+1][i]
+        const int xr = cj * S::FXW + (ci + 1);
+        const int yr = (cj + 1) * TX + ci;
         const bool xlo_open = (gi == 0) && !px;
         const bool ylo_open = (gj == 0) && !py;
         double U[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             double v = g.uin[q][k];
-            double fr = sFX[q * S::FX_SZ + xr];
+            double fr = sFX[q * S::FX4_SZ + xr];
             if (xlo_open) v = v - dtdx * fr;
-            else v = v - dtdx * fr + dtdx * sFX[q * S::FX_SZ + xr - 1];
-            double ft = sFY[q * S::FY_SZ + yr];
+            else v = v - dtdx * fr + dtdx * sFX[q * S::FX4_SZ + xr - 1];
+            double ft = sFY[q * S::FY4_SZ + yr];
             if (ylo_open) v = v - dtdx * ft;
-            else v = v - dtdx * ft + dtdx * sFY[q * S::FY_SZ + yr - S::FYW];
+            else v = v - dtdx * ft + dtdx * sFY[q * S::FY4_SZ + yr - TX];
             U[q] = v;
             if (STORE != STORE_NONE) store->Up[q][k] = v;
         }
@@ -359,8 +371,8 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
 
         // corner EMF: Ez(a,b) = 1/4 (-F_x(By)(a,b) - F_x(By)(a,b+1) + F_y(Bx)(a,b) + F_y(Bx)(a+1,b))
         auto negEz = [&](int a, int b) -> double {
-            const double *fbx = sFX + 4 * S::FX_SZ;
-            const double *fby = sFY + 4 * S::FY_SZ;
+            const double *fbx = sBX;
+            const double *fby = sBY;
             double f0 = fbx[(b + 1) * S::FXW + (a + 1)];
             double up = (!py && y0 + b == ny - 1) ? f0 : fbx[(b + 2) * S::FXW + (a + 1)];
             double e0 = fby[(b + 1) * S::FYW + (a + 1)];

@@ -15,6 +15,9 @@ from . import _lib
 from ._lib import BC, Scheme, check
 
 DT_REFERENCE, DT_CFL, DT_PRESCRIBED = 0, 1, 2
+# tile shapes of the fused stage kernel (CHIMERA_CTX_NUM_SHAPES), in shape-index order
+TILE_SHAPES = ((32, 8), (16, 16), (64, 4), (32, 32), (32, 16))
+NUM_TILE_SHAPES = len(TILE_SHAPES)
 
 
 def _host_ptrs(arr: np.ndarray):
@@ -136,7 +139,7 @@ class Grid:
 
     def autotune(self, reps: int = 3) -> tuple[int, list[float]]:
         best = C.c_int()
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * NUM_TILE_SHAPES)()
         check(self._L.chimera_ctx_autotune(self._h, int(reps), C.byref(best), ms), "autotune")
         return best.value, list(ms)
 
