@@ -15,7 +15,7 @@ from . import _lib
 from ._lib import BC, Scheme, check
 
 DT_REFERENCE, DT_CFL = 0, 1
-SHAPES = {0: (32, 8), 1: (16, 16), 2: (64, 4)}
+SHAPES = {0: (32, 8), 1: (16, 16), 2: (64, 4), 3: (32, 32), 4: (32, 16)}  # 3: not HLLD
 
 
 class Ensemble:

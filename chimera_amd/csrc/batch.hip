@@ -38,8 +38,12 @@ __device__ __forceinline__ void xcd_segment(int ntiles, int &begin, int &end, in
     end = (int)((long long)ntiles * (xcd + 1) / 8);
 }
 
+// A 512-thread block (32 x 16) is two waves per SIMD, so two blocks per CU need <= 128
+// VGPRs (Rusanov and HLLE need about that; HLLD needs more and keeps one block).
 template <int SOLVER, int LIMITER, int TX, int TY>
-__global__ __launch_bounds__(STAGE_THREADS) void batch_stage_kernel(
+__global__ __launch_bounds__(TX * TY)
+__attribute__((amdgpu_waves_per_eu((TX * TY == 512 && SOLVER != HLLD) ? 4 : 1)))
+void batch_stage_kernel(
     const MemberDesc *__restrict__ mem, const TileRef *__restrict__ tiles, int ntiles, int stage,
     int cur, int use_lim, int mood) {
     __shared__ double lds[TileShape<TX, TY>::LDS_DOUBLES];
@@ -74,10 +78,11 @@ __global__ __launch_bounds__(STAGE_THREADS) void batch_stage_kernel(
 
 // Per-member max of cf + |v| over the tiles of the u^n state (src/main.f90:69-79).
 template <int TX, int TY>
-__global__ __launch_bounds__(STAGE_THREADS) void batch_cfl_kernel(const MemberDesc *__restrict__ mem,
-                                                                  const TileRef *__restrict__ tiles,
-                                                                  int ntiles, int cur) {
-    __shared__ double part[STAGE_THREADS / 64];
+__global__ __launch_bounds__(TX * TY) void batch_cfl_kernel(const MemberDesc *__restrict__ mem,
+                                                            const TileRef *__restrict__ tiles,
+                                                            int ntiles, int cur) {
+    constexpr int NW = TX * TY / 64;   // waves per block
+    __shared__ double part[NW];
     int begin, end, slot, slots;
     xcd_segment(ntiles, begin, end, slot, slots);
     for (int t = begin + slot; t < end; t += slots) {
@@ -102,7 +107,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void batch_cfl_kernel(const MemberDe
         __syncthreads();
         if (threadIdx.x == 0) {
             double mx = part[0];
-            for (int w = 1; w < STAGE_THREADS / 64; ++w) mx = fmax(mx, part[w]);
+            for (int w = 1; w < NW; ++w) mx = fmax(mx, part[w]);
             atomicMax(&m.st->smax_bits, encode_nonneg(mx));
         }
         __syncthreads();
@@ -146,23 +151,35 @@ __global__ void batch_init_kernel(StepState *st, int members) {
 using BStageFn = void (*)(const MemberDesc *, const TileRef *, int, int, int, int, int);
 using BCflFn = void (*)(const MemberDesc *, const TileRef *, int, int);
 
+// The single-grid driver's tile shapes (RES_TILE_SHAPES, same indices and the same
+// availability: no 32 x 32 for HLLD); nullptr: shape not available for that solver.
 #define CH_BSHAPES(S, L)                                                                   \
     {batch_stage_kernel<S, L, 32, 8>, batch_stage_kernel<S, L, 16, 16>,                    \
-     batch_stage_kernel<S, L, 64, 4>}
+     batch_stage_kernel<S, L, 64, 4>, batch_stage_kernel<S, L, 32, 32>,                    \
+     batch_stage_kernel<S, L, 32, 16>}
+#define CH_BSHAPES_HLLD(S, L)                                                              \
+    {batch_stage_kernel<S, L, 32, 8>, batch_stage_kernel<S, L, 16, 16>,                    \
+     batch_stage_kernel<S, L, 64, 4>, nullptr, batch_stage_kernel<S, L, 32, 16>}
 
 static BStageFn bstage_fn(int solver, int limiter, int shape) {
-    static const BStageFn table[3][2][NUM_SHAPES] = {
+    static const BStageFn table[3][2][RES_NUM_SHAPES] = {
         {CH_BSHAPES(RUSANOV, LIM_VAN_LEER), CH_BSHAPES(RUSANOV, LIM_MC)},
         {CH_BSHAPES(HLLE, LIM_VAN_LEER), CH_BSHAPES(HLLE, LIM_MC)},
-        {CH_BSHAPES(HLLD, LIM_VAN_LEER), CH_BSHAPES(HLLD, LIM_MC)}};
+        {CH_BSHAPES_HLLD(HLLD, LIM_VAN_LEER), CH_BSHAPES_HLLD(HLLD, LIM_MC)}};
     return table[solver - 1][limiter - 1][shape];
 }
 #undef CH_BSHAPES
+#undef CH_BSHAPES_HLLD
 
 static BCflFn bcfl_fn(int shape) {
-    static const BCflFn table[NUM_SHAPES] = {batch_cfl_kernel<32, 8>, batch_cfl_kernel<16, 16>,
-                                             batch_cfl_kernel<64, 4>};
+    static const BCflFn table[RES_NUM_SHAPES] = {
+        batch_cfl_kernel<32, 8>, batch_cfl_kernel<16, 16>, batch_cfl_kernel<64, 4>,
+        batch_cfl_kernel<32, 32>, batch_cfl_kernel<32, 16>};
     return table[shape];
+}
+
+static int shape_threads(int shape) {
+    return RES_TILE_SHAPES[shape].tx * RES_TILE_SHAPES[shape].ty;
 }
 
 }  // namespace chimera
@@ -179,8 +196,8 @@ struct chimera_batch {
     StepState *st = nullptr;
     double *arena = nullptr;
     long long cells = 0;
-    TileRef *dtiles[NUM_SHAPES] = {};
-    int ntiles[NUM_SHAPES] = {};
+    TileRef *dtiles[RES_NUM_SHAPES] = {};
+    int ntiles[RES_NUM_SHAPES] = {};
     int shape = 0;
     int blocks_per_cu = 0;  // 0: occupancy-derived
     int grid = 0;
@@ -208,7 +225,8 @@ static int batch_set_grid(chimera_batch *b) {
     if (bpc <= 0) {
         int occ = 0;
         BStageFn fn = bstage_fn(b->sch.riemann_solver, b->sch.slope_limiter, b->shape);
-        CH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, STAGE_THREADS, 0));
+        CH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn,
+                                                            shape_threads(b->shape), 0));
         bpc = occ > 0 ? occ : 1;
     }
     int g = b->cu_count * bpc;
@@ -225,8 +243,8 @@ static int batch_tables(chimera_batch *b) {
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
         return (long long)b->hmem[a].nx * b->hmem[a].ny > (long long)b->hmem[c].nx * b->hmem[c].ny;
     });
-    for (int sh = 0; sh < NUM_SHAPES; ++sh) {
-        const TileDims d = TILE_SHAPES[sh];
+    for (int sh = 0; sh < RES_NUM_SHAPES; ++sh) {
+        const TileDims d = RES_TILE_SHAPES[sh];
         std::vector<TileRef> t;
         for (int m : order) {
             int tx = (b->hmem[m].nx + d.tx - 1) / d.tx, ty = (b->hmem[m].ny + d.ty - 1) / d.ty;
@@ -242,7 +260,7 @@ static int batch_tables(chimera_batch *b) {
 
 static int batch_launch_stage(chimera_batch *b, int stage) {
     BStageFn fn = bstage_fn(b->sch.riemann_solver, b->sch.slope_limiter, b->shape);
-    hipLaunchKernelGGL(fn, dim3(b->grid), dim3(STAGE_THREADS), 0, b->stream, b->dmem,
+    hipLaunchKernelGGL(fn, dim3(b->grid), dim3(shape_threads(b->shape)), 0, b->stream, b->dmem,
                        b->dtiles[b->shape], b->ntiles[b->shape], stage, b->cur,
                        b->sch.use_slope_limiting, b->sch.fallback_2_mood);
     return launch_check("batch_stage_kernel");
@@ -250,7 +268,8 @@ static int batch_launch_stage(chimera_batch *b, int stage) {
 
 static int batch_enqueue_step(chimera_batch *b, int mode, bool timed) {
     const int mb = (b->members + 127) / 128;
-    hipLaunchKernelGGL(bcfl_fn(b->shape), dim3(b->grid), dim3(STAGE_THREADS), 0, b->stream,
+    hipLaunchKernelGGL(bcfl_fn(b->shape), dim3(b->grid), dim3(shape_threads(b->shape)), 0,
+                       b->stream,
                        b->dmem, b->dtiles[b->shape], b->ntiles[b->shape], b->cur);
     CH_TRY(launch_check("batch_cfl_kernel"));
     hipLaunchKernelGGL(batch_dt_kernel, dim3(mb), dim3(128), 0, b->stream, b->dmem, b->members, mode);
@@ -463,7 +482,11 @@ extern "C" int chimera_batch_cells(chimera_batch_t *b, long long *cells) {
 
 extern "C" int chimera_batch_set_tiling(chimera_batch_t *b, int shape, int blocks_per_cu) {
     CH_REQUIRE(b, "batch is NULL");
-    CH_REQUIRE(shape >= 0 && shape < NUM_SHAPES, "tile shape must be 0..%d", NUM_SHAPES - 1);
+    CH_REQUIRE(shape >= 0 && shape < RES_NUM_SHAPES, "tile shape must be 0..%d",
+               RES_NUM_SHAPES - 1);
+    CH_REQUIRE(bstage_fn(b->sch.riemann_solver, b->sch.slope_limiter, shape),
+               "tile shape %d is not available for riemann_solver %d", shape,
+               b->sch.riemann_solver);
     CH_REQUIRE(blocks_per_cu >= 0 && blocks_per_cu <= 16, "blocks_per_cu must be 0..16");
     CH_HIP(hipStreamSynchronize(b->stream));
     b->shape = shape;
@@ -492,10 +515,12 @@ extern "C" int chimera_batch_autotune(chimera_batch_t *b, int reps, int *best_sh
     double best = 1e300;
     int bs = b->shape, bb = b->blocks_per_cu;
     const int saved_cur = b->cur;
-    for (int sh = 0; sh < NUM_SHAPES; ++sh) {
+    for (int sh = 0; sh < RES_NUM_SHAPES; ++sh) {
         int occ = 0;
         BStageFn fn = bstage_fn(b->sch.riemann_solver, b->sch.slope_limiter, sh);
-        CH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, STAGE_THREADS, 0));
+        if (!fn) continue;
+        CH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn,
+                                                            shape_threads(sh), 0));
         for (int bpc = 1; bpc <= std::max(1, occ); ++bpc) {
             b->shape = sh;
             b->blocks_per_cu = bpc;

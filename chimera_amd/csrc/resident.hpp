@@ -10,24 +10,24 @@
 
 namespace chimera {
 
-// Tile shapes the stage kernels are instantiated for; every shape has 256 cells, one
-// per thread in the update phase.  32 x 8 keeps 46 KiB of LDS (primitives of the
-// 36 x 12 halo tile plus 33 x 10 x-face and 34 x 9 y-face fluxes).
+// Tile shapes of the stage kernels (single grid and batched ensemble).  One thread per
+// cell in the update phase.  The 256-cell shapes 32 x 8, 16 x 16 and 64 x 4 keep
+// 41-49 KiB of LDS (32 x 8: primitives of the 36 x 12 halo tile, the four conserved
+// fluxes of 33 x 8 x faces and 32 x 9 y faces, the transverse B flux of 33 x 10 x faces
+// and 34 x 9 y faces), three blocks per CU.  32 x 32 is run by 1024 threads (Rusanov
+// and HLLE; HLLD needs more than the 128 VGPRs a 1024-thread block leaves each lane):
+// per cell 1.27 primitive conversions and 2.19 Riemann faces instead of 1.69 and 2.48
+// for 32 x 8, in 144 KiB of LDS, one block (16 waves) per CU.  32 x 16 is run by 512
+// threads (1.41 conversions, 2.29 faces); its 76 KiB of LDS let two blocks share a CU,
+// so that one block's barrier waits overlap the other's work.
 struct TileDims {
     int tx, ty;
 };
-constexpr int NUM_SHAPES = 3;
-constexpr TileDims TILE_SHAPES[NUM_SHAPES] = {{32, 8}, {16, 16}, {64, 4}};
-constexpr int STAGE_THREADS = 256;
-// The single-grid driver also has a 32 x 32 tile run by 1024 threads (Rusanov and
-// HLLE; HLLD needs more than the 128 VGPRs a 1024-thread block leaves each lane): per
-// cell 1.27 primitive conversions and 2.19 Riemann faces instead of 1.69 and 2.48 for
-// 32 x 8, in 144 KiB of LDS, one block (16 waves) per CU; and a 32 x 16 tile run by 512
-// threads (1.41 conversions, 2.29 faces) whose 76 KiB of LDS let two blocks share a
-// CU, so that one block's barrier waits overlap the other's work.
 constexpr int RES_NUM_SHAPES = 5;
 constexpr TileDims RES_TILE_SHAPES[RES_NUM_SHAPES] = {
     {32, 8}, {16, 16}, {64, 4}, {32, 32}, {32, 16}};
+// block size of the adjoint's forward-stage kernels (256-cell tiles)
+constexpr int STAGE_THREADS = 256;
 
 // Per-grid step bookkeeping kept on the device so whole steps can be captured.
 struct StepState {

@@ -20,7 +20,7 @@ def _members():
 
 
 @pytest.mark.parametrize("graph", [False, True])
-@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 def test_batch_members_match_oracle(gpu_lib, orc, graph, shape):
     from chimera_amd.batch import Ensemble, DT_CFL
     probs = _members()
@@ -42,7 +42,7 @@ def test_batch_reference_dt_policy_and_autotune(gpu_lib, orc):
     probs = _members()[:4]
     with Ensemble(probs) as ens:
         best = ens.autotune(reps=2)
-        assert 0 <= best[0] <= 2 and best[1] >= 1
+        assert 0 <= best[0] <= 4 and best[1] >= 1
         for m, p in enumerate(probs):                 # autotuning leaves the state untouched
             assert np.array_equal(ens.download(m), p.state)
         ens.steps(3, dt_mode=DT_REFERENCE, graph=True)
@@ -80,6 +80,23 @@ def test_single_grid_autotune_keeps_state(gpu_lib):
         best, ms = g.autotune(reps=2)
         assert 0 <= best <= 4 and len(ms) == 5 and all(t > 0 for t in ms)
         assert np.array_equal(g.download(), prob.state)
+
+
+def test_batch_hlld_tiles(gpu_lib, orc):
+    """HLLD members: the 32 x 16 tile matches the oracle; 32 x 32 is refused."""
+    from chimera_amd._lib import ChimeraError
+    from chimera_amd.batch import Ensemble, DT_CFL
+    probs = [problems.grf_member(n, 2000 + k, scheme=(3, 2, 1, 1)) for k, n in enumerate([40, 72])]
+    with Ensemble(probs) as ens:
+        with pytest.raises(ChimeraError):
+            ens.set_tiling(3, 0)
+        ens.set_tiling(4, 0)
+        ens.steps(3, dt_mode=DT_CFL)
+        outs = [ens.download(m) for m in range(len(probs))]
+    for m, p in enumerate(probs):
+        cpu = p.state.copy()
+        orc.run(cpu, p.dx, p.gamma, p.bc, list(p.scheme), 3, dt_mode=1)
+        assert np.array_equal(outs[m], cpu), f"member {m}"
 
 
 def test_large_tile_unavailable_for_hlld(gpu_lib):
