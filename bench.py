@@ -128,6 +128,7 @@ def batched_grf(args, torch) -> dict:
         times = []
         for m in idx:
             with Grid.from_problem(probs[m]) as g:
+                g.autotune(reps=2)                 # the plain path at its best tile shape
                 g.steps(2, dt_mode=DT_CFL, graph=True)
                 g.sync()
                 t1 = time.perf_counter()
@@ -210,6 +211,7 @@ def adjoint_field_loop(args, torch) -> dict:
     g = Grid.from_problem(prob)
     T = run_forward(g, prob.dx, Az_t, dts)
     g.upload(field_loop_state(torch.from_numpy(Az).cuda(), prob.dx))
+    g.autotune(reps=3)                         # the forward at its best tile shape
     g.steps(2, dt_mode=DT_PRESCRIBED, dt=dts[:2], graph=True)
     g.sync()
     g.upload(field_loop_state(torch.from_numpy(Az).cuda(), prob.dx))
@@ -217,6 +219,7 @@ def adjoint_field_loop(args, torch) -> dict:
     g.steps(nsteps, dt_mode=DT_PRESCRIBED, dt=dts, graph=True)
     g.sync()
     t_fwd = time.perf_counter() - t0
+    fwd_tile = list(g.tile_shape()[1:])
     g.close()
     gradient(prob, Az, dts[:K], T)   # warm-up: the backward's workspace for one segment
     grad, loss_v, t_evolve, t_bwd = gradient(prob, Az, dts, T)
@@ -287,7 +290,7 @@ def adjoint_field_loop(args, torch) -> dict:
                         f"(dt frozen from a CFL run), checkpoint every {K}",
             "loss": loss_v, "grad_nodes": int(grad.size), "grad_finite": grad_finite,
             "forward_s": t_fwd, "evolve_forward_s": t_evolve, "backward_s": t_bwd,
-            "cost_ratio": t_bwd / t_fwd, "fd_check": fd}
+            "cost_ratio": t_bwd / t_fwd, "forward_tile": fwd_tile, "fd_check": fd}
 
 
 def shard_strip(args, torch, full_stage_ms: float) -> dict:

@@ -55,6 +55,9 @@ __device__ __forceinline__ int wrap_or_clamp(int g, int n, int periodic) {
 
 // ---- forward recompute: the fused stage kernel, keeping the intermediates -----------
 
+// tile of the recompute kernel: 32 x 8 (a 32 x 16 tile, faster for the plain stage
+// kernel, made the STORE_UP recompute slower on the 512 x 256 field loop: 174 vs 168 ms
+// per 1000-step backward)
 constexpr int ATX = 32, ATY = 8;
 // tile shapes of the fused reverse kernels (chimera_adj_set_tiles); smaller tiles give
 // more workgroups on small grids at the price of more halo work
@@ -65,7 +68,7 @@ constexpr int REV_NUM_SHAPES = 4;
 constexpr RevTile REV_SHAPES[REV_NUM_SHAPES] = {{16, 16}, {16, 8}, {8, 8}, {32, 8}};
 
 template <int SOLVER, int LIMITER, int STORE>
-__global__ __launch_bounds__(STAGE_THREADS) void adj_stage_kernel(GridView g, StageStore st,
+__global__ __launch_bounds__(ATX * ATY) void adj_stage_kernel(GridView g, StageStore st,
                                                                  int stage, int use_lim, int mood,
                                                                  int tiles_x, int ntiles) {
     __shared__ double lds[TileShape<ATX, ATY>::LDS_DOUBLES];
@@ -545,7 +548,7 @@ int run_forward_stage(chimera_adj *a, int stage, const Ptr6 &in, const Ptr6 &u0,
     if (mode == STORE_UP)
         for (int q = 0; q < 4; ++q) store.Up[q] = up[q];
     hipLaunchKernelGGL(adj_stage_fn(a->sch.riemann_solver, a->sch.slope_limiter, mode),
-                       dim3(a->grid), dim3(STAGE_THREADS), 0, st ? st : a->stream, g, store, stage,
+                       dim3(a->grid), dim3(ATX * ATY), 0, st ? st : a->stream, g, store, stage,
                        a->G.use_lim, a->G.mood, a->tiles_x, a->ntiles);
     return launch_check("adjoint forward stage");
 }
@@ -734,14 +737,15 @@ Ptr4 up_slot(chimera_adj *a, int s) {
 }
 Tab up_tab(chimera_adj *a, int s) { return (Tab)(a->up_tab + 4 * (size_t)s); }
 
-// Recompute one checkpoint segment from h0 on stream st, keeping every stage input in
-// the trajectory slots from base on -- slot base+3s holds U^s (s >= 1; U^0 is h0),
-// slots base+3s+1 and base+3s+2 the stage outputs U^(1), U^(2), slot base+3K the last
-// step's output -- and the Up of step s, stage st in Up slot upbase+3s+st-1.  dt_dev:
-// the segment's dts.
-int recompute_segment(chimera_adj *a, hipStream_t st, const Ptr6 &h0, const double *dt_dev,
-                      int nsteps, int base, int upbase) {
-    for (int s = 0; s < nsteps; ++s) {
+// Recompute steps [s_begin, s_end) of a checkpoint segment that starts at h0, on
+// stream st, keeping every stage input in the trajectory slots from base on -- slot
+// base+3s holds U^s (s >= 1; U^0 is h0), slots base+3s+1 and base+3s+2 the stage
+// outputs U^(1), U^(2), slot base+3K the last step's output -- and the Up of step s,
+// stage st in Up slot upbase+3s+st-1.  dt_dev: the segment's dts.  Step s reads slot
+// base+3s, so the steps must be enqueued in order.
+int recompute_steps(chimera_adj *a, hipStream_t st, const Ptr6 &h0, const double *dt_dev,
+                    int s_begin, int s_end, int base, int upbase) {
+    for (int s = s_begin; s < s_end; ++s) {
         const Ptr6 in = s == 0 ? h0 : traj_slot(a, base + 3 * s);
         const double *dts = dt_dev + s;
         for (int stage = 1; stage <= 3; ++stage) {
@@ -753,14 +757,18 @@ int recompute_segment(chimera_adj *a, hipStream_t st, const Ptr6 &h0, const doub
     }
     return CHIMERA_OK;
 }
+int recompute_segment(chimera_adj *a, hipStream_t st, const Ptr6 &h0, const double *dt_dev,
+                      int nsteps, int base, int upbase) {
+    return recompute_steps(a, st, h0, dt_dev, 0, nsteps, base, upbase);
+}
 
-// Reverse of a recomputed segment on a->stream (h0 / t0: its checkpoint as host pointer
-// set and device table).  lam ping-pongs between cur and other; on return cur holds
-// lam at the segment's start.
-int reverse_segment(chimera_adj *a, const Ptr6 &h0, Tab t0, const double *dt_dev,
-                    const double *dt_host, int nsteps, int base, int upbase, Tab &cur,
-                    Tab &other) {
-    for (int s = nsteps - 1; s >= 0; --s) {
+// The reverse of steps s_end-1 down to s_begin of a segment whose trajectory is in
+// the slots from base / upbase; cur holds lam of step s_end on entry and lam of step
+// s_begin on return (cur and other swap once per step).
+int reverse_steps(chimera_adj *a, const Ptr6 &h0, Tab t0, const double *dt_dev,
+                  const double *dt_host, int s_begin, int s_end, int base, int upbase, Tab &cur,
+                  Tab &other) {
+    for (int s = s_end - 1; s >= s_begin; --s) {
         const Ptr6 hs = s == 0 ? h0 : traj_slot(a, base + 3 * s);
         Tab ts = s == 0 ? t0 : slot_tab(a, base + 3 * s);
         const Ptr6 s1 = traj_slot(a, base + 3 * s + 1), s2 = traj_slot(a, base + 3 * s + 2);
@@ -773,6 +781,11 @@ int reverse_segment(chimera_adj *a, const Ptr6 &h0, Tab t0, const double *dt_dev
         other = t;
     }
     return CHIMERA_OK;
+}
+int reverse_segment(chimera_adj *a, const Ptr6 &h0, Tab t0, const double *dt_dev,
+                    const double *dt_host, int nsteps, int base, int upbase, Tab &cur,
+                    Tab &other) {
+    return reverse_steps(a, h0, t0, dt_dev, dt_host, 0, nsteps, base, upbase, cur, other);
 }
 
 // lam back into the caller's arrays when the ping-pong ended in lam_c
@@ -1057,30 +1070,38 @@ extern "C" int chimera_adj_sweep(chimera_adj_t *a, const double *const *states, 
     // the recompute stream starts after everything queued on the main stream
     CH_HIP(hipEventRecord(a->ev_start, a->stream));
     CH_HIP(hipStreamWaitEvent(a->rstream, a->ev_start, 0));
-    auto recompute = [&](int k) -> int {
+    // one step s of segment k: its recompute (on the second stream, into half k & 1) and
+    // its reverse (on the main stream)
+    auto rec_step = [&](int k, int s) -> int {
         const int b = k & 1;
-        CH_TRY(recompute_segment(a, a->rstream, ptr6(states + 6 * (size_t)k),
-                                 a->dt_dev + (size_t)k * seg_steps, seg_len(k), b * (3 * K + 1),
-                                 b * 3 * K));
-        CH_HIP(hipEventRecord(a->ev_traj[b], a->rstream));
-        return CHIMERA_OK;
+        return recompute_steps(a, a->rstream, ptr6(states + 6 * (size_t)k),
+                               a->dt_dev + (size_t)k * seg_steps, s, s + 1, b * (3 * K + 1),
+                               b * 3 * K);
     };
     Tab cur = table(a, TB_LIN), other = table(a, TB_LC);
-    if (have_last) {
-        CH_HIP(hipEventRecord(a->ev_traj[(nseg - 1) & 1], a->rstream));   // already there
-    } else {
-        CH_TRY(recompute(nseg - 1));
-    }
+    auto rev_step = [&](int k, int s) -> int {
+        const int b = k & 1;
+        return reverse_steps(a, ptr6(states + 6 * (size_t)k), (Tab)(a->ck_tab + 6 * (size_t)k),
+                             a->dt_dev + (size_t)k * seg_steps, dt_in + (size_t)k * seg_steps, s,
+                             s + 1, b * (3 * K + 1), b * 3 * K, cur, other);
+    };
+    if (!have_last)   // else the last segment's trajectory is already there
+        for (int s = 0; s < seg_len(nseg - 1); ++s) CH_TRY(rec_step(nseg - 1, s));
+    CH_HIP(hipEventRecord(a->ev_traj[(nseg - 1) & 1], a->rstream));
     for (int k = nseg - 1; k >= 0; --k) {
-        if (k >= 1) {
-            // segment k-1 goes into the half segment k+1 used: after that reverse
-            if (k + 1 <= nseg - 1) CH_HIP(hipStreamWaitEvent(a->rstream, a->ev_rev[(k + 1) & 1], 0));
-            CH_TRY(recompute(k - 1));
-        }
         CH_HIP(hipStreamWaitEvent(a->stream, a->ev_traj[k & 1], 0));
-        CH_TRY(reverse_segment(a, ptr6(states + 6 * (size_t)k), (Tab)(a->ck_tab + 6 * (size_t)k),
-                               a->dt_dev + (size_t)k * seg_steps, dt_in + (size_t)k * seg_steps,
-                               seg_len(k), (k & 1) * (3 * K + 1), (k & 1) * 3 * K, cur, other));
+        // segment k-1 goes into the half segment k+1 used: after that reverse
+        if (k >= 1 && k + 1 <= nseg - 1)
+            CH_HIP(hipStreamWaitEvent(a->rstream, a->ev_rev[(k + 1) & 1], 0));
+        // the reverse of segment k (steps in descending order) and the recompute of
+        // segment k-1 are enqueued step by step in turn, so that both streams always
+        // have work queued whatever the host's launch rate
+        const int nrev = seg_len(k), nrec = k >= 1 ? seg_len(k - 1) : 0;
+        for (int i = 0; i < std::max(nrev, nrec); ++i) {
+            if (i < nrev) CH_TRY(rev_step(k, nrev - 1 - i));
+            if (i < nrec) CH_TRY(rec_step(k - 1, i));
+        }
+        if (k >= 1) CH_HIP(hipEventRecord(a->ev_traj[(k - 1) & 1], a->rstream));
         CH_HIP(hipEventRecord(a->ev_rev[k & 1], a->stream));
     }
     CH_TRY(finish_lam(a, cur, lam));
