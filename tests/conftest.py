@@ -1,6 +1,7 @@
 """Shared fixtures.  GPU tests are marked ``gpu``; everything else runs on the CPU."""
 from __future__ import annotations
 
+import os
 import subprocess
 import sys
 from pathlib import Path
@@ -57,6 +58,14 @@ def _h5_name(problem_type, n, seed, extra) -> str:
     return f"p{problem_type}_N{n}_s{seed}{tag}.h5"
 
 
+def ref_env() -> dict:
+    """Environment for the reference's OpenMP executables: idle threads sleep instead of
+    spinning, so that they do not starve other test processes on a shared CPU."""
+    env = dict(os.environ)
+    env.setdefault("OMP_WAIT_POLICY", "passive")
+    return env
+
+
 @pytest.fixture(scope="session")
 def ref_main_run(tmp_path_factory):
     """run(problem_type, N, seed=0, extra=None) -> HDF5 path of the reference's own main
@@ -73,7 +82,7 @@ def ref_main_run(tmp_path_factory):
             name = _h5_name(*key)
             argv = [str(REF_MAIN), str(problem_type), str(n), str(seed), f"{base}/", name]
             argv += [f"{x:g}" for x in extra or ()]
-            r = subprocess.run(argv, capture_output=True, text=True, timeout=900)
+            r = subprocess.run(argv, capture_output=True, text=True, timeout=900, env=ref_env())
             assert r.returncode == 0, r.stderr
             cache[key], stdout[key] = base / name, r.stdout
         return cache[key]

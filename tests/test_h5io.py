@@ -17,6 +17,8 @@ import pytest
 
 from chimera_amd import cli, h5io
 
+from tests.conftest import ref_env
+
 ROOT = Path(__file__).resolve().parent.parent
 REF_MAIN = ROOT / "oracle" / "_ref" / "chimera_ref_main"
 ATHENA_H5 = Path("/root/reference/tests/validation/ot_reference.h5")
@@ -112,7 +114,8 @@ def ref_ot32(tmp_path_factory):
     if not REF_MAIN.exists():
         pytest.skip("oracle/_ref/chimera_ref_main is not built")
     out = tmp_path_factory.mktemp("refmain")
-    r = subprocess.run([str(REF_MAIN), "1", "32", "0", f"{out}/", "ot32.h5"], capture_output=True,
+    r = subprocess.run([str(REF_MAIN), "1", "32", "0", f"{out}/", "ot32.h5"], env=ref_env(),
+                       capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     return out / "ot32.h5", r.stdout
@@ -188,7 +191,8 @@ def test_executable_wrapper(tmp_path):
 @pytest.mark.skipif(not REF_MAIN.exists(), reason="oracle/_ref/chimera_ref_main is not built")
 def test_cli_messages_match_reference_main():
     for argv in (["1", "32"], ["6", "32", "0"], ["1", "2", "0"], ["1", "32", "q"]):
-        r = subprocess.run([str(REF_MAIN), *argv], capture_output=True, text=True, timeout=60)
+        r = subprocess.run([str(REF_MAIN), *argv], capture_output=True, text=True, timeout=60,
+                           env=ref_env())
         assert r.returncode == 0
         with pytest.raises(cli.UsageError) as e:
             cli.parse_args(argv)
