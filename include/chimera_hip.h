@@ -201,7 +201,8 @@ int chimera_ctx_set_timing(chimera_ctx_t *ctx, int enable);
 int chimera_ctx_stage_kernel_ms(chimera_ctx_t *ctx, double *ms, int *launches);
 /* Tile shape of the fused stage kernel: 0 = 32x8, 1 = 16x16, 2 = 64x4 cells (256
  * threads), 3 = 32x32 cells (1024 threads; Rusanov and HLLE only), 4 = 32x16 cells
- * (512 threads). */
+ * (512 threads).  A new context uses 4 for Rusanov and HLLE on grids of at least one
+ * 32x16 tile per CU, 0 otherwise.  Every shape gives the same bits. */
 #define CHIMERA_CTX_NUM_SHAPES 5
 int chimera_ctx_set_tile_shape(chimera_ctx_t *ctx, int shape);
 int chimera_ctx_get_tile_shape(chimera_ctx_t *ctx, int *shape, int *tx, int *ty);
