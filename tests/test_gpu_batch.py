@@ -73,6 +73,23 @@ def test_single_grid_tile_shapes(gpu_lib, shape, scheme):
     assert np.array_equal(out, ref)
 
 
+@pytest.mark.parametrize("make,scheme", [("rotor", (2, 1, 1, 1)), ("orszag_tang", (1, 2, 1, 1)),
+                                         ("rotor", (3, 2, 1, 1))])
+def test_default_tile_on_large_grids_matches_oracle(gpu_lib, orc, make, scheme):
+    """A 512^2 grid has a 32 x 16 tile per CU, so a new context runs Rusanov and HLLE
+    on 32 x 16 tiles (HLLD keeps 32 x 8); two CFL steps equal the oracle's bit for bit."""
+    from chimera_amd.solver import Grid
+    prob = getattr(problems, make)(512, scheme=scheme)
+    print(f"default tile {make} scheme={scheme} 512^2", flush=True)
+    with Grid.from_problem(prob) as g:
+        assert g.tile_shape()[0] == (0 if scheme[0] == 3 else 4)
+        g.steps(2, dt_mode=1)
+        out = g.download()
+    ref = prob.state.copy()
+    orc.run(ref, prob.dx, prob.gamma, prob.bc, list(prob.scheme), 2, dt_mode=1)
+    assert np.array_equal(out, ref)
+
+
 def test_single_grid_autotune_keeps_state(gpu_lib):
     from chimera_amd.solver import Grid
     prob = problems.orszag_tang(128)
