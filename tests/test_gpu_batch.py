@@ -90,6 +90,24 @@ def test_default_tile_on_large_grids_matches_oracle(gpu_lib, orc, make, scheme):
     assert np.array_equal(out, ref)
 
 
+def test_batch_large_members_autotuned_match_oracle(gpu_lib, orc):
+    """Members of the benchmark's sizes (512^2 and 256^2 GRF, Rusanov / MC) at the tile
+    and blocks-per-CU count the autotuner picks: two CFL steps equal the oracle's run
+    of each member alone, bit for bit."""
+    from chimera_amd.batch import Ensemble, DT_CFL
+    probs = [problems.grf_member(n, 3000 + k, scheme=(1, 2, 1, 1)) for k, n in enumerate([512, 256])]
+    print("batch large members 512^2 + 256^2, autotuned", flush=True)
+    with Ensemble(probs) as ens:
+        shape, bpc = ens.autotune(reps=1)
+        print(f"autotuned tile {shape} blocks/CU {bpc}", flush=True)
+        ens.steps(2, dt_mode=DT_CFL, graph=True)
+        outs = [ens.download(m) for m in range(len(probs))]
+    for m, p in enumerate(probs):
+        ref = p.state.copy()
+        orc.run(ref, p.dx, p.gamma, p.bc, list(p.scheme), 2, dt_mode=1)
+        assert np.array_equal(outs[m], ref), f"member {m}"
+
+
 def test_single_grid_autotune_keeps_state(gpu_lib):
     from chimera_amd.solver import Grid
     prob = problems.orszag_tang(128)
