@@ -862,6 +862,8 @@ extern "C" int chimera_adj_create(chimera_adj_t **out, int nx, int ny, double dx
                                   void *stream) {
     CH_REQUIRE(out, "out is NULL");
     CH_REQUIRE(nx >= 4 && ny >= 4, "grid must be at least 4 x 4");
+    CH_REQUIRE((long long)nx * ny <= MAX_GRID_CELLS, "grid of %d x %d cells exceeds %lld cells", nx, ny,
+               MAX_GRID_CELLS);
     CH_REQUIRE(dx > 0.0 && gamma > 1.0, "dx must be > 0 and gamma > 1");
     int px, py;
     CH_TRY(check_bc(bc, &px, &py));

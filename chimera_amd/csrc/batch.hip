@@ -331,6 +331,8 @@ extern "C" int chimera_batch_create(chimera_batch_t **out, int members, const in
     long long cells = 0;
     for (int m = 0; m < members; ++m) {
         CH_REQUIRE(nx[m] >= 4 && ny[m] >= 4, "member %d: grid must be at least 4 x 4", m);
+        CH_REQUIRE((long long)nx[m] * ny[m] <= MAX_GRID_CELLS, "member %d: grid exceeds %lld cells", m,
+                   MAX_GRID_CELLS);
         CH_REQUIRE(dx[m] > 0.0 && gamma[m] > 1.0, "member %d: dx must be > 0, gamma > 1", m);
         int px, py;
         CH_TRY(check_bc(&bc[m], &px, &py));

@@ -434,8 +434,10 @@ extern "C" int chimera_badj_create(chimera_badj_t **out, int members, const int 
     b->uptraj.assign(members, nullptr);
     size_t total = 0;
     for (int m = 0; m < members; ++m) {
-        if (!(nx[m] >= 4 && ny[m] >= 4 && dx[m] > 0.0 && gamma[m] > 1.0)) {
-            set_error("member %d: need nx, ny >= 4, dx > 0 and gamma > 1", m);
+        if (!(nx[m] >= 4 && ny[m] >= 4 && dx[m] > 0.0 && gamma[m] > 1.0 &&
+              (long long)nx[m] * ny[m] <= MAX_GRID_CELLS)) {
+            set_error("member %d: need nx, ny >= 4, nx * ny <= %lld, dx > 0 and gamma > 1", m,
+                      MAX_GRID_CELLS);
             delete b;
             return CHIMERA_EINVAL;
         }

@@ -57,7 +57,29 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(GridView g, int stage, i
                                         tile / tiles_x, lds);
 }
 
+// Stage 1 with the MOOD / thermal-check bits of every owned face ORed into g.mask
+// (chimera_ctx_stage1_mask); 32 x 8 tiles.
+template <int SOLVER, int LIMITER>
+__global__ __launch_bounds__(256) void stage_mask_kernel(GridView g, int use_lim, int mood,
+                                                          int tiles_x, int ntiles) {
+    __shared__ double lds[TileShape<32, 8>::LDS_DOUBLES];
+    const int b = blockIdx.x, nb = gridDim.x;  // nb is a multiple of 8
+    const int tile = (b % 8) * (nb / 8) + b / 8;
+    if (tile >= ntiles) return;
+    stage_tile<SOLVER, LIMITER, 32, 8, STORE_NONE, true>(g, 1, use_lim != 0, mood != 0,
+                                                          tile % tiles_x, tile / tiles_x, lds);
+}
+
 using StageFn = void (*)(GridView, int, int, int, int, int);
+using MaskFn = void (*)(GridView, int, int, int, int);
+
+MaskFn mask_fn(int solver, int limiter) {
+    static const MaskFn table[3][2] = {
+        {stage_mask_kernel<RUSANOV, LIM_VAN_LEER>, stage_mask_kernel<RUSANOV, LIM_MC>},
+        {stage_mask_kernel<HLLE, LIM_VAN_LEER>, stage_mask_kernel<HLLE, LIM_MC>},
+        {stage_mask_kernel<HLLD, LIM_VAN_LEER>, stage_mask_kernel<HLLD, LIM_MC>}};
+    return table[solver - 1][limiter - 1];
+}
 
 #define CH_SHAPES(S, L)                                                                 \
     {stage_kernel<S, L, 32, 8>, stage_kernel<S, L, 16, 16>, stage_kernel<S, L, 64, 4>,     \
@@ -192,6 +214,12 @@ __global__ __launch_bounds__(256) void divb_reduce_kernel(const double *bx, cons
 }
 
 int launch_stage(const StageLaunch &L, hipStream_t stream) {
+    if (L.g.mask) {   // the diagnostics variant (stage 1, 32 x 8 tiles)
+        const int tiles_x = (L.g.nx + 31) / 32, ntiles = tiles_x * ((L.g.ny + 7) / 8);
+        hipLaunchKernelGGL(mask_fn(L.solver, L.limiter), dim3((ntiles + 7) / 8 * 8), dim3(256), 0,
+                           stream, L.g, L.use_lim, L.mood, tiles_x, ntiles);
+        return launch_check("stage_mask_kernel");
+    }
     StageFn fn = stage_fn(L.solver, L.limiter, L.shape);
     CH_REQUIRE(fn, "tile shape %d is not available for riemann_solver %d", L.shape, L.solver);
     const TileDims d = RES_TILE_SHAPES[L.shape];
@@ -388,6 +416,8 @@ extern "C" int chimera_ctx_create(chimera_ctx_t **out, int nx, int ny, double dx
                                   void *stream) {
     CH_REQUIRE(out, "ctx out-pointer is NULL");
     CH_REQUIRE(nx >= 4 && ny >= 4, "grid must be at least 4 x 4 (got %d x %d)", nx, ny);
+    CH_REQUIRE((long long)nx * ny <= MAX_GRID_CELLS, "grid of %d x %d cells exceeds %lld cells", nx, ny,
+               MAX_GRID_CELLS);
     CH_REQUIRE(dx > 0.0 && gamma > 1.0, "dx must be > 0 and gamma > 1");
     int px, py;
     CH_TRY(check_bc(bc, &px, &py));

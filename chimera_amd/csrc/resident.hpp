@@ -26,6 +26,9 @@ struct TileDims {
 constexpr int RES_NUM_SHAPES = 5;
 constexpr TileDims RES_TILE_SHAPES[RES_NUM_SHAPES] = {
     {32, 8}, {16, 16}, {64, 4}, {32, 32}, {32, 16}};
+// Largest grid (cells per field) of the fused kernels: they address a field with 32-bit
+// byte offsets.
+constexpr long long MAX_GRID_CELLS = 1ll << 29;
 // block size of the adjoint's forward-stage kernels (256-cell tiles)
 constexpr int STAGE_THREADS = 256;
 

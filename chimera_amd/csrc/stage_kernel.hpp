@@ -22,6 +22,7 @@
 #pragma once
 
 #include <cstdint>
+#include <type_traits>
 
 #include "mhd_phys.hpp"
 
@@ -199,12 +200,129 @@ __device__ __forceinline__ bool thermal_reset(double f[6], const double c[6]) {
     return false;
 }
 
-template <int SOLVER, int LIMITER, int TX, int TY, int STORE = STORE_NONE>
+// Raw FP64 min / max (v_min_f64 / v_max_f64).  fmin / fmax of values the compiler
+// cannot prove canonical (LDS loads, selects of them) get a canonicalising v_max of
+// each operand first, to quiet signalling NaNs; the stage never holds a signalling
+// NaN, and for every other input these are fmin / fmax exactly (IEEE mode: a quiet
+// NaN operand yields the other one).  The MOOD bounds and face floors of a face use
+// 16-20 of them.
+__device__ __forceinline__ double vmin(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double vmax(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// the same with a wave-uniform second operand (a constant) held in SGPRs
+__device__ __forceinline__ double vmax_s(double a, double s) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "s"(s));
+    return r;
+}
+
+// Face states of one field at one face without the diagnostic bits: the MUSCL left
+// state, the right state, MOOD against the 5-point stencil of the left cell (Q1: the
+// right state too) and the rho/P floor (src/mhd_derivatives.f90:150-229).  At a
+// non-periodic hi boundary the reference takes the cell value itself for the right
+// state; here the clamped halo makes p = pp = c there, so the right cell's slope is
+// exactly 0 (limited: dfL * dfR = 0; central: (c - c) / 2dx) and p - hdx * 0 = c,
+// the same bits without a test.
+__device__ __forceinline__ void face_states_nb(double m, double c, double p, double nA, double nB,
+                                               double sl, double sr, bool mood, bool rho_or_p,
+                                               double hdx, double &fl, double &fr) {
+    double l = c + hdx * sl;
+    double r = p - hdx * sr;
+    if (mood) {
+        double lo = vmin(vmin(vmin(vmin(c, p), m), nA), nB);
+        const double hi = vmax(vmax(vmax(vmax(c, p), m), nA), nB);
+        if (rho_or_p) lo = vmax_s(lo, 1e-9);
+        if (l < lo || l > hi) l = c;
+        if (r < lo || r > hi) r = c;
+    }
+    if (rho_or_p) {
+        l = vmax_s(l, 1.0e-8);
+        r = vmax_s(r, 1.0e-8);
+    }
+    fl = l;
+    fr = r;
+}
+
+// Global loads / stores at a 32-bit byte offset from a wave-uniform base (the
+// scalar-base form of the global instructions: no 64-bit address arithmetic per
+// access).  Grids of the fused path stay below 2^29 cells per field.
+__device__ __forceinline__ double ldg(const double *base, uint32_t off) {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+}
+__device__ __forceinline__ void stg(double *base, uint32_t off, double v) {
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+// True when v is a positive power of two (division by it is then exactly a
+// multiplication by 1 / v, for every dividend).
+__device__ __forceinline__ bool pow2_value(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long e = (b >> 52) & 0x7ffull;
+    return (b >> 63) == 0 && e != 0 && e != 0x7ff && (b & 0xfffffffffffffull) == 0;
+}
+
+// get_primitive (src/mhd_change_states.f90:91-110) with the two divisions by the cell
+// volume done as multiplications when vol is a power of two (the same bits).
+template <bool VOL_POW2>
+__device__ __forceinline__ void cons_to_prim_v(double M, double Mx, double My, double E, double Bx,
+                                               double By, double g, double vol, double ivol,
+                                               double &rho, double &vx, double &vy, double &P) {
+    double r = fmax(VOL_POW2 ? M * ivol : M / vol, RHO_FLOOR);
+    double u = Mx / (r * vol);
+    double w = My / (r * vol);
+    double halfB2 = 0.5 * (Bx * Bx + By * By);
+    double ke = 0.5 * r * (u * u + w * w);
+    double p = ((VOL_POW2 ? E * ivol : E / vol) - ke - halfB2) * (g - 1.0) + halfB2;
+    p = fmax(p, P_FLOOR);
+    p = fmax(p, halfB2 + E_FLOOR_FRAC * (ke + halfB2));
+    rho = r;
+    vx = u;
+    vy = w;
+    P = p;
+}
+
+// apply_conserved_floors (src/mhd_change_states.f90:143-164) as conserved_floors, with
+// the velocity cap's square root and division skipped when |Mom|^2 < 49^2 M^2: then
+// |Mom| / M < 49 (1 + 2^-53)^2 after every rounding, so the cap test is false and the
+// result is unchanged.  A NaN or infinite |Mom|^2 fails the pre-test and takes the
+// reference's own test.
+__device__ __forceinline__ void conserved_floors_fast(double &M, double &Mx, double &My, double &E,
+                                                      double Bx, double By, double vol, double g) {
+    const double V_MAX = 50.0;
+    if (M < RHO_FLOOR * vol) {
+        double halfB2 = 0.5 * (Bx * Bx + By * By);
+        Mx = 0.0;
+        My = 0.0;
+        M = RHO_FLOOR * vol;
+        E = fmax(E, (P_FLOOR / (g - 1.0) + halfB2) * vol);
+    }
+    const double s2 = Mx * Mx + My * My;
+    if (!(s2 < 2401.0 * (M * M))) {
+        double mom = sqrt(s2);
+        if (mom / M > V_MAX) {
+            double scale = V_MAX * M / mom;
+            Mx = Mx * scale;
+            My = My * scale;
+        }
+    }
+}
+
+// MASK: also OR the MOOD / thermal bits of owned faces into g.mask (the stage-1
+// diagnostics); the bits are assembled only with MASK or STORE_ALL.
+template <int SOLVER, int LIMITER, int TX, int TY, int STORE = STORE_NONE, bool MASK = false>
 __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool use_lim, bool mood,
                                            int tile_x, int tile_y, double *lds,
                                            const StageStore *store = nullptr) {
     using S = TileShape<TX, TY>;
     constexpr int NT = TX * TY;
+    constexpr bool BITS = MASK || STORE == STORE_ALL;
     const int tid = threadIdx.x;
     const int nx = g.nx, ny = g.ny, px = g.px, py = g.py;
     const int x0 = tile_x * TX, y0 = tile_y * TY;
@@ -219,40 +337,47 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
     const double hdx = 0.5 * dx;
 
     // ---- phase 1: halo load and stage primitives -------------------------------
-    for (int idx = tid; idx < S::P_SZ; idx += NT) {
-        int lx = idx % S::PW, ly = idx / S::PW;
-        int gi = map_index(x0 - 2 + lx, nx, px);
-        int gj = map_index(y0 - 2 + ly, ny, py);
-        int gim = gi > 0 ? gi - 1 : (px ? nx - 1 : 0);
-        int gjm = gj > 0 ? gj - 1 : (py ? ny - 1 : 0);
-        size_t k = (size_t)gi + (size_t)nx * gj;
-        double Bx = 0.5 * (g.uin[4][k] + g.uin[4][(size_t)gim + (size_t)nx * gj]);
-        double By = 0.5 * (g.uin[5][k] + g.uin[5][(size_t)gi + (size_t)nx * gjm]);
-        double rho, vx, vy, P;
-        cons_to_prim(g.uin[0][k], g.uin[1][k], g.uin[2][k], g.uin[3][k], Bx, By, gam, vol, rho, vx,
-                     vy, P);
-        sP[0 * S::P_SZ + idx] = rho;
-        sP[1 * S::P_SZ + idx] = vx;
-        sP[2 * S::P_SZ + idx] = vy;
-        sP[3 * S::P_SZ + idx] = P;
-        sP[4 * S::P_SZ + idx] = Bx;
-        sP[5 * S::P_SZ + idx] = By;
-        if (STORE == STORE_ALL) {
-            int ci = lx - 2, cj = ly - 2;
-            if (ci >= 0 && ci < TX && cj >= 0 && cj < TY && x0 + ci < nx && y0 + cj < ny) {
-                size_t ko = (size_t)(x0 + ci) + (size_t)nx * (y0 + cj);
-                store->P[0][ko] = rho;
-                store->P[1][ko] = vx;
-                store->P[2][ko] = vy;
-                store->P[3][ko] = P;
-                store->P[4][ko] = Bx;
-                store->P[5][ko] = By;
+    auto phase1 = [&](auto vol_pow2) {
+        constexpr bool P2 = decltype(vol_pow2)::value;
+        const double ivol = 1.0 / vol;   // exact when vol is a power of two
+        for (int idx = tid; idx < S::P_SZ; idx += NT) {
+            const int lx = idx % S::PW, ly = idx / S::PW;
+            const int gi = map_index(x0 - 2 + lx, nx, px);
+            const int gj = map_index(y0 - 2 + ly, ny, py);
+            const int gim = gi > 0 ? gi - 1 : (px ? nx - 1 : 0);
+            const int gjm = gj > 0 ? gj - 1 : (py ? ny - 1 : 0);
+            const uint32_t row = (uint32_t)nx * (uint32_t)gj;
+            const uint32_t k = (row + (uint32_t)gi) * 8u;
+            const uint32_t kim = (row + (uint32_t)gim) * 8u;
+            const uint32_t kjm = ((uint32_t)nx * (uint32_t)gjm + (uint32_t)gi) * 8u;
+            const double Bx = 0.5 * (ldg(g.uin[4], k) + ldg(g.uin[4], kim));
+            const double By = 0.5 * (ldg(g.uin[5], k) + ldg(g.uin[5], kjm));
+            double rho, vx, vy, P;
+            cons_to_prim_v<P2>(ldg(g.uin[0], k), ldg(g.uin[1], k), ldg(g.uin[2], k),
+                               ldg(g.uin[3], k), Bx, By, gam, vol, ivol, rho, vx, vy, P);
+            sP[0 * S::P_SZ + idx] = rho;
+            sP[1 * S::P_SZ + idx] = vx;
+            sP[2 * S::P_SZ + idx] = vy;
+            sP[3 * S::P_SZ + idx] = P;
+            sP[4 * S::P_SZ + idx] = Bx;
+            sP[5 * S::P_SZ + idx] = By;
+            if (STORE == STORE_ALL) {
+                int ci = lx - 2, cj = ly - 2;
+                if (ci >= 0 && ci < TX && cj >= 0 && cj < TY && x0 + ci < nx && y0 + cj < ny) {
+                    size_t ko = (size_t)(x0 + ci) + (size_t)nx * (y0 + cj);
+                    store->P[0][ko] = rho;
+                    store->P[1][ko] = vx;
+                    store->P[2][ko] = vy;
+                    store->P[3][ko] = P;
+                    store->P[4][ko] = Bx;
+                    store->P[5][ko] = By;
+                }
             }
         }
-    }
+    };
+    if (pow2_value(vol)) phase1(std::true_type{});
+    else phase1(std::false_type{});
     __syncthreads();
-
-    const bool want_mask = g.mask != nullptr;
 
     // ---- phase 2: every x face and y face the tile needs, as runs of consecutive faces
     //      along their normal (one run per thread): the slope of a run's right cell is
@@ -264,57 +389,88 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         const int dir = run.dir;                        // 0: x faces, 1: y faces (rotated)
         const int sA = dir ? S::PW : 1;                 // LDS stride along the face normal
         const int sT = dir ? 1 : S::PW;                 // LDS stride across it
+        // x face: i = fx-1 in row j = fy-1, left cell at LDS (fx+1, fy+1);
+        // y face: column i = fx-1, j = fy-1, lower cell at LDS (fx+1, fy+1)
+        const int fx0 = dir ? run.line : run.start, fy0 = dir ? run.start : run.line;
+        const double *b0 = sP + (fy0 + 1) * S::PW + (fx0 + 1);   // left cell, advances by sA
+        // transverse-flux slot of the face (every face) and its step along the run
+        double *bt = dir ? sBY + fy0 * S::FYW + fx0 : sBX + fy0 * S::FXW + fx0;
+        const int bt_step = dir ? S::FYW : 1;
+        // conserved-flux slots: faces of the tile's own rows (x) / columns (y) only; the
+        // normal / tangential momentum fluxes go to Momx / Momy by direction
+        const bool cons = run.line >= 1 && run.line <= (dir ? TX : TY);
+        const int fsz = dir ? S::FY4_SZ : S::FX4_SZ;
+        double *cf = dir ? sFY + fy0 * TX + (fx0 - 1) : sFX + (fy0 - 1) * S::FXW + fx0;
+        const int cf_step = dir ? TX : 1;
+        const int off_n = dir ? 2 * fsz : fsz, off_t = dir ? fsz : 2 * fsz;
         // slope of the first face's left cell; afterwards each face's right-cell slope
         // is carried to the next face
         double sprev[6];
         if (run.len > 0) {
-            const int fx0 = dir ? run.line : run.start, fy0 = dir ? run.start : run.line;
-            const int lpos0 = (fy0 + 1) * S::PW + (fx0 + 1);
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double *b = sP + q * S::P_SZ + lpos0;
+                const double *b = b0 + q * S::P_SZ;
                 sprev[q] = cell_slope<LIMITER>(b[-sA], b[0], b[sA], use_lim, inv_dx, inv_2dx);
             }
         }
         for (int t = 0; t < run.len; ++t) {
-            // x face: i = fx-1 in row j = fy-1, left cell at LDS (fx+1, fy+1);
-            // y face: column i = fx-1, j = fy-1, lower cell at LDS (fx+1, fy+1)
-            const int fx = dir ? run.line : run.start + t;
-            const int fy = dir ? run.start + t : run.line;
-            const int lpos = (fy + 1) * S::PW + (fx + 1);
-            const bool last = dir ? (!py && map_index(y0 + fy - 1, ny, py) == ny - 1)
-                                  : (!px && map_index(x0 + fx - 1, nx, px) == nx - 1);
+            const double *bf = b0 + t * sA;
             double L[6], R[6], C[6];
             unsigned bits = 0, fbits = 0;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double *b = sP + q * S::P_SZ + lpos;
+                const double *b = bf + q * S::P_SZ;
                 const double m = b[-sA], c = b[0], p = b[sA], pp = b[2 * sA];
                 C[q] = c;
                 const double sl = sprev[q];
                 const double sr = cell_slope<LIMITER>(c, p, pp, use_lim, inv_dx, inv_2dx);
                 sprev[q] = sr;
-                unsigned fb;
-                bits |= face_states(m, c, p, b[sT], b[-sT], sl, sr, last, mood,
-                                    q == F_RHO || q == F_P, hdx, L[q], R[q], fb)
-                        << (4 * q + 2 * dir);
-                if (q == F_RHO) fbits |= fb << (2 * dir);
-                if (q == F_P) fbits |= fb << (4 + 2 * dir);
-            }
-            if (thermal_reset(L, C)) bits |= 1u << (24 + 2 * dir);
-            if (thermal_reset(R, C)) bits |= 1u << (25 + 2 * dir);
-            const int i = fx - 1, j = fy - 1;
-            const bool owned = i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny;
-            if (STORE == STORE_ALL && owned) {
-                size_t ko = (size_t)(x0 + i) + (size_t)nx * (y0 + j);
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    store->face[4 * q + 2 * dir][ko] = L[q];
-                    store->face[4 * q + 2 * dir + 1][ko] = R[q];
+                if (BITS) {
+                    unsigned fb;
+                    bits |= face_states(m, c, p, b[sT], b[-sT], sl, sr, false, mood,
+                                        q == F_RHO || q == F_P, hdx, L[q], R[q], fb)
+                            << (4 * q + 2 * dir);
+                    if (q == F_RHO) fbits |= fb << (2 * dir);
+                    if (q == F_P) fbits |= fb << (4 + 2 * dir);
+                } else {
+                    face_states_nb(m, c, p, b[sT], b[-sT], sl, sr, mood, q == F_RHO || q == F_P,
+                                   hdx, L[q], R[q]);
                 }
-                unsigned long long w = (unsigned long long)bits | ((unsigned long long)fbits << 32);
-                if (dir) store->bits_y[ko] = w;
-                else store->bits_x[ko] = w;
+            }
+            // thermal-pressure check (src/mhd_derivatives.f90:262-267): both faces reset
+            // to the cell values below P_floor; rare, so the selects run only in waves
+            // where some face resets
+            {
+                const double pthL = L[F_P] - 0.5 * (L[F_BX] * L[F_BX] + L[F_BY] * L[F_BY]);
+                const double pthR = R[F_P] - 0.5 * (R[F_BX] * R[F_BX] + R[F_BY] * R[F_BY]);
+                const bool tl = pthL < P_FLOOR, tr = pthR < P_FLOOR;
+                if (BITS) bits |= (tl ? 1u << (24 + 2 * dir) : 0u) | (tr ? 1u << (25 + 2 * dir) : 0u);
+                if (__builtin_amdgcn_ballot_w64(tl || tr)) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        if (tl) L[q] = C[q];
+                        if (tr) R[q] = C[q];
+                    }
+                }
+            }
+            const int fx = dir ? run.line : run.start + t;
+            const int fy = dir ? run.start + t : run.line;
+            const int i = fx - 1, j = fy - 1;
+            if (STORE == STORE_ALL || MASK) {
+                const bool owned = i >= 0 && i < TX && j >= 0 && j < TY && x0 + i < nx && y0 + j < ny;
+                if (STORE == STORE_ALL && owned) {
+                    size_t ko = (size_t)(x0 + i) + (size_t)nx * (y0 + j);
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        store->face[4 * q + 2 * dir][ko] = L[q];
+                        store->face[4 * q + 2 * dir + 1][ko] = R[q];
+                    }
+                    unsigned long long w = (unsigned long long)bits | ((unsigned long long)fbits << 32);
+                    if (dir) store->bits_y[ko] = w;
+                    else store->bits_x[ko] = w;
+                }
+                if (MASK && owned && bits)
+                    atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
             }
             FaceState a{L[F_RHO], dir ? L[F_VY] : L[F_VX], dir ? L[F_VX] : L[F_VY], L[F_P],
                         dir ? L[F_BY] : L[F_BX], dir ? L[F_BX] : L[F_BY]};
@@ -322,19 +478,15 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                          dir ? R[F_BY] : R[F_BX], dir ? R[F_BX] : R[F_BY]};
             Flux5 f = riemann<SOLVER>(a, bR, gam);
             // transverse flux: By flux (x faces) / Bx flux (y faces), every face
-            (dir ? sBY : sBX)[dir ? fy * S::FYW + fx : fy * S::FXW + fx] = f.bt;
+            bt[t * bt_step] = f.bt;
             // conserved fluxes: faces of the tile's own rows (x) / columns (y)
-            if (dir ? (fx >= 1 && fx <= TX) : (fy >= 1 && fy <= TY)) {
-                double *sF = dir ? sFY : sFX;
-                const int fsz = dir ? S::FY4_SZ : S::FX4_SZ;
-                const int idx = dir ? fy * TX + (fx - 1) : (fy - 1) * S::FXW + fx;
-                sF[0 * fsz + idx] = f.mass;
-                sF[1 * fsz + idx] = dir ? f.mom_t : f.mom_n;   // Momx
-                sF[2 * fsz + idx] = dir ? f.mom_n : f.mom_t;   // Momy
-                sF[3 * fsz + idx] = f.energy;
+            if (cons) {
+                double *c0 = cf + t * cf_step;
+                c0[0] = f.mass;
+                c0[off_n] = f.mom_n;
+                c0[off_t] = f.mom_t;
+                c0[3 * fsz] = f.energy;
             }
-            if (want_mask && owned && bits)
-                atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
         }
     }
     __syncthreads();
@@ -344,7 +496,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         int ci = c % TX, cj = c / TX;
         int gi = x0 + ci, gj = y0 + cj;
         if (gi >= nx || gj >= ny) continue;
-        size_t k = (size_t)gi + (size_t)nx * gj;
+        const uint32_t k = ((uint32_t)nx * (uint32_t)gj + (uint32_t)gi) * 8u;
         const double dt = *g.dt;
         const double dtdx = dt * dx;
         // conserved fluxes: x faces FX(i, j) at [j][i+1], y faces FY(i, j) at [j+1][i]
@@ -355,7 +507,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         double U[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            double v = g.uin[q][k];
+            double v = ldg(g.uin[q], k);
             double fr = sFX[q * S::FX4_SZ + xr];
             if (xlo_open) v = v - dtdx * fr;
             else v = v - dtdx * fr + dtdx * sFX[q * S::FX4_SZ + xr - 1];
@@ -363,11 +515,11 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             if (ylo_open) v = v - dtdx * ft;
             else v = v - dtdx * ft + dtdx * sFY[q * S::FY4_SZ + yr - TX];
             U[q] = v;
-            if (STORE != STORE_NONE) store->Up[q][k] = v;
+            if (STORE != STORE_NONE) stg(store->Up[q], k, v);
         }
         const int pc = (cj + 2) * S::PW + (ci + 2);
         const double Bxc = sP[4 * S::P_SZ + pc], Byc = sP[5 * S::P_SZ + pc];
-        conserved_floors(U[0], U[1], U[2], U[3], Bxc, Byc, vol, gam);
+        conserved_floors_fast(U[0], U[1], U[2], U[3], Bxc, Byc, vol, gam);
 
         // corner EMF: Ez(a,b) = 1/4 (-F_x(By)(a,b) - F_x(By)(a,b+1) + F_y(Bx)(a,b) + F_y(Bx)(a+1,b))
         auto negEz = [&](int a, int b) -> double {
@@ -385,8 +537,8 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         else dbx = (negEz(ci, cj) - negEz(ci, cj - 1)) * inv_dx;
         if (xlo_open) dby = -((negEz(ci + 1, cj) - negEz(ci, cj)) * inv_dx);
         else dby = -((negEz(ci, cj) - negEz(ci - 1, cj)) * inv_dx);
-        double bxn = g.uin[4][k] + dt * dbx;
-        double byn = g.uin[5][k] + dt * dby;
+        double bxn = ldg(g.uin[4], k) + dt * dbx;
+        double byn = ldg(g.uin[5], k) + dt * dby;
 
         // outflow boundary flux correction (src/main.f90:116-148), stage primitives
         if (xlo_open || ylo_open) {
@@ -410,21 +562,21 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
 
         if (STORE == STORE_ALL) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) store->U3[q][k] = U[q];
+            for (int q = 0; q < 4; ++q) stg(store->U3[q], k, U[q]);
         }
         if (stage > 1) {
             const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
             const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) U[q] = c0 * g.u0[q][k] + c1 * U[q];
-            bxn = c0 * g.u0[4][k] + c1 * bxn;
-            byn = c0 * g.u0[5][k] + c1 * byn;
-            conserved_floors(U[0], U[1], U[2], U[3], Bxc, Byc, vol, gam);
+            for (int q = 0; q < 4; ++q) U[q] = c0 * ldg(g.u0[q], k) + c1 * U[q];
+            bxn = c0 * ldg(g.u0[4], k) + c1 * bxn;
+            byn = c0 * ldg(g.u0[5], k) + c1 * byn;
+            conserved_floors_fast(U[0], U[1], U[2], U[3], Bxc, Byc, vol, gam);
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) g.uout[q][k] = U[q];
-        g.uout[4][k] = bxn;
-        g.uout[5][k] = byn;
+        for (int q = 0; q < 4; ++q) stg(g.uout[q], k, U[q]);
+        stg(g.uout[4], k, bxn);
+        stg(g.uout[5], k, byn);
     }
 }
 
