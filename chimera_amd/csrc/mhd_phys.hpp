@@ -91,18 +91,23 @@ __device__ __forceinline__ double cfl_speed(double rho, double vx, double vy, do
     return cf + sqrt(vx * vx + vy * vy);
 }
 
-// One limited slope from three consecutive values,
-// src/mhd_derivatives.f90:82-97 (MC / Van Leer, zero at extrema and flat spots).
+// The slope limiter of src/mhd_derivatives.f90:82-97 (MC / Van Leer, zero at extrema
+// and flat spots) from the one-sided differences dfL = (f0 - fm) / dx and
+// dfR = (fp - f0) / dx; a cell's dfR is its right neighbour's dfL, bit for bit.
 template <int LIMITER>
-__device__ __forceinline__ double limited_slope(double fm, double f0, double fp, double inv_dx) {
-    double dfL = (f0 - fm) * inv_dx;
-    double dfR = (fp - f0) * inv_dx;
+__device__ __forceinline__ double limiter_of(double dfL, double dfR) {
     double prod = dfL * dfR;
     double den = dfL + dfR;
     if (prod <= 0.0 || fabs(den) < 1e-12) return 0.0;
     if (LIMITER == LIM_VAN_LEER) return (2.0 * prod) / den;
     double m = fmin(fmin(2.0 * fabs(dfL), 2.0 * fabs(dfR)), 0.5 * fabs(den));
     return copysign(1.0, dfL) * m;
+}
+
+// One limited slope from three consecutive values.
+template <int LIMITER>
+__device__ __forceinline__ double limited_slope(double fm, double f0, double fp, double inv_dx) {
+    return limiter_of<LIMITER>((f0 - fm) * inv_dx, (fp - f0) * inv_dx);
 }
 
 // Central slope of compute_gradients, src/mhd_derivatives.f90:37-45 (used only

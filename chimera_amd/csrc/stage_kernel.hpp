@@ -57,7 +57,10 @@ struct StageStore {
 // all the flux difference reads.
 template <int TX, int TY>
 struct TileShape {
-    static constexpr int PW = TX + 4, PH = TY + 4;     // primitives with a 2-cell halo
+    // primitives of cells [-2, TX+2) x [-2, TY+2) (a 2-cell halo); rows PW apart, PW odd
+    // so that the rows of neighbouring face runs fall into different LDS banks
+    static constexpr int LW = TX + 4, PH = TY + 4;
+    static constexpr int PW = LW | 1;
     static constexpr int FXW = TX + 1, FXH = TY + 2;   // x faces i in [-1,TX-1], j in [-1,TY]
     static constexpr int FYW = TX + 2, FYH = TY + 1;   // y faces i in [-1,TX],   j in [-1,TY-1]
     static constexpr int P_SZ = PW * PH;
@@ -167,19 +170,25 @@ struct FaceRunTable {
             ++cnt[best];
             ++total;
         }
+        // order: run length (longest first), direction, index k of the run within its
+        // line, line -- so neighbouring lanes take the same run of neighbouring lines
+        // (LDS addresses one pitch or one cell apart: few bank conflicts)
+        int kmax = 0;
+        for (int l = 0; l < NL; ++l) kmax = cnt[l] > kmax ? cnt[l] : kmax;
         int t = 0;
         for (int want = lmax; want >= 1; --want)
-            for (int l = 0; l < NL; ++l) {
-                int m = len_of(l), c = cnt[l], base = m / c, extra = m % c, pos = 0;
-                for (int k = 0; k < c; ++k) {
-                    int len = base + (k < extra ? 1 : 0);
-                    if (len == want && t < NT) {
-                        r[t] = FaceRun{l < NXL ? 0 : 1, l < NXL ? l : l - NXL, pos, len};
-                        ++t;
+            for (int dir = 0; dir < 2; ++dir)
+                for (int k = 0; k < kmax; ++k)
+                    for (int l = dir ? NXL : 0; l < (dir ? NL : NXL); ++l) {
+                        const int m = len_of(l), c = cnt[l], base = m / c, extra = m % c;
+                        if (k >= c) continue;
+                        const int len = base + (k < extra ? 1 : 0);
+                        const int pos = k * base + (k < extra ? k : extra);
+                        if (len == want && t < NT) {
+                            r[t] = FaceRun{dir, dir ? l - NXL : l, pos, len};
+                            ++t;
+                        }
                     }
-                    pos += len;
-                }
-            }
         for (; t < NT; ++t) r[t] = FaceRun{0, 0, 0, 0};
     }
     static constexpr int len_of(int l) { return l < NXL ? XLEN : YLEN; }
@@ -323,7 +332,11 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
     using S = TileShape<TX, TY>;
     constexpr int NT = TX * TY;
     constexpr bool BITS = MASK || STORE == STORE_ALL;
-    const int tid = threadIdx.x;
+    // the thread index made opaque per call: in a persistent loop over tiles the
+    // compiler would otherwise keep every thread-dependent value of the tile (run
+    // table entry, LDS pointers, strides) live in registers across the whole loop
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const int nx = g.nx, ny = g.ny, px = g.px, py = g.py;
     const int x0 = tile_x * TX, y0 = tile_y * TY;
     double *sP = lds;                       // [6][PH][PW]
@@ -340,8 +353,9 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
     auto phase1 = [&](auto vol_pow2) {
         constexpr bool P2 = decltype(vol_pow2)::value;
         const double ivol = 1.0 / vol;   // exact when vol is a power of two
-        for (int idx = tid; idx < S::P_SZ; idx += NT) {
-            const int lx = idx % S::PW, ly = idx / S::PW;
+        for (int n = tid; n < S::LW * S::PH; n += NT) {
+            const int lx = n % S::LW, ly = n / S::LW;
+            const int idx = ly * S::PW + lx;
             const int gi = map_index(x0 - 2 + lx, nx, px);
             const int gj = map_index(y0 - 2 + ly, ny, py);
             const int gim = gi > 0 ? gi - 1 : (px ? nx - 1 : 0);
@@ -404,13 +418,20 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         const int cf_step = dir ? TX : 1;
         const int off_n = dir ? 2 * fsz : fsz, off_t = dir ? fsz : 2 * fsz;
         // slope of the first face's left cell; afterwards each face's right-cell slope
-        // is carried to the next face
-        double sprev[6];
+        // is carried to the next face, and with limiting the right cell's one-sided
+        // difference dfR, which is the next right cell's dfL
+        double sprev[6], dprev[6];
         if (run.len > 0) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 const double *b = b0 + q * S::P_SZ;
-                sprev[q] = cell_slope<LIMITER>(b[-sA], b[0], b[sA], use_lim, inv_dx, inv_2dx);
+                if (use_lim) {
+                    const double dR = (b[sA] - b[0]) * inv_dx;
+                    sprev[q] = limiter_of<LIMITER>((b[0] - b[-sA]) * inv_dx, dR);
+                    dprev[q] = dR;
+                } else {
+                    sprev[q] = central_slope(b[-sA], b[sA], inv_2dx);
+                }
             }
         }
         for (int t = 0; t < run.len; ++t) {
@@ -423,7 +444,14 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 const double m = b[-sA], c = b[0], p = b[sA], pp = b[2 * sA];
                 C[q] = c;
                 const double sl = sprev[q];
-                const double sr = cell_slope<LIMITER>(c, p, pp, use_lim, inv_dx, inv_2dx);
+                double sr;
+                if (use_lim) {
+                    const double dR = (pp - p) * inv_dx;
+                    sr = limiter_of<LIMITER>(dprev[q], dR);
+                    dprev[q] = dR;
+                } else {
+                    sr = central_slope(c, pp, inv_2dx);
+                }
                 sprev[q] = sr;
                 if (BITS) {
                     unsigned fb;
@@ -491,11 +519,11 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
     }
     __syncthreads();
 
-    // ---- phase 3: update, floors, CT, outflow correction, blend -----------------
-    for (int c = tid; c < TX * TY; c += NT) {
-        int ci = c % TX, cj = c / TX;
-        int gi = x0 + ci, gj = y0 + cj;
-        if (gi >= nx || gj >= ny) continue;
+    // ---- phase 3: update, floors, CT, outflow correction, blend (one cell per thread)
+    {
+        const int ci = tid % TX, cj = tid / TX;
+        const int gi = x0 + ci, gj = y0 + cj;
+        if (gi >= nx || gj >= ny) return;
         const uint32_t k = ((uint32_t)nx * (uint32_t)gj + (uint32_t)gi) * 8u;
         const double dt = *g.dt;
         const double dtdx = dt * dx;

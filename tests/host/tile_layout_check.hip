@@ -1,7 +1,7 @@
 // Host check of the fused stage kernel's tile bookkeeping (stage_kernel.hpp), for every
 // tile shape the kernels are built for: the compile-time face-run table covers every x
 // and y face of the tile exactly once, every LDS read of phase 2 stays inside the
-// primitive array, every conserved-flux slot of the trimmed layout is written exactly
+// primitive array (and out of the pad column of its odd row pitch), every conserved-flux slot of the trimmed layout is written exactly
 // once, and the reads of phase 3 (flux difference, corner EMFs, including the open
 // lo-side cases) stay inside what phase 2 wrote.  Built and run by
 // tests/test_tile_layout.py; exit status 0 when every check holds.
@@ -20,7 +20,8 @@ int check() {
         const FaceRun run = T.r[t];
         const int dir = run.dir;
         const int sA = dir ? S::PW : 1, sT = dir ? 1 : S::PW;
-        auto inP = [&](int p) { if (p < 0 || p >= S::P_SZ) { ++err; } };
+        // inside the array and never in the pad column of the odd row pitch
+        auto inP = [&](int p) { if (p < 0 || p >= S::P_SZ || p % S::PW >= S::LW) { ++err; } };
         if (run.len > 0) {
             const int fx0 = dir ? run.line : run.start, fy0 = dir ? run.start : run.line;
             const int lpos0 = (fy0 + 1) * S::PW + (fx0 + 1);
