@@ -145,6 +145,7 @@ def _declare(L: C.CDLL) -> None:
         "chimera_badj_record": [vp, C.POINTER(vp), c_dp, i, i, i, C.POINTER(vp)],
         "chimera_badj_set_tiles": [vp, i, i, i],
         "chimera_badj_autotune": [vp, C.POINTER(vp), c_dp, i, C.POINTER(i), C.POINTER(C.c_double)],
+        "chimera_step_noalloc": [C.POINTER(vp), d, C.POINTER(vp), i, i, d, d, pBC, pSch, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -10,7 +10,7 @@
  * success and a negative chimera_status on failure; chimera_last_error() describes
  * the last failure on the calling thread.
  *
- * Three layers:
+ * Layers:
  *   1. Drop-ins: one function per reference routine, same argument meaning and order
  *      as the Fortran, plus the BC flags / scheme choices the Fortran reads from
  *      module mhd_config.  Host pointers; data is copied to the GPU and back per
@@ -18,6 +18,7 @@
  *   2. Resident single-grid API (chimera_ctx_*): state lives in HBM, whole SSP-RK3
  *      steps run as fused LDS-tiled stage kernels, dt is reduced on the device.
  *   3. Batched ensemble (chimera_batch_*) and adjoint (chimera_adj_*).
+ *   4. An allocation-free step (chimera_step_noalloc).
  */
 #ifndef CHIMERA_HIP_H
 #define CHIMERA_HIP_H
@@ -361,6 +362,25 @@ int chimera_badj_set_tiles(chimera_badj_t *badj, int shape_a, int shape_b, int s
 int chimera_badj_autotune(chimera_badj_t *badj, const double *const *state0, const double *dt_in,
                           int reps, int *best, double *ms);
 void *chimera_badj_stream(chimera_badj_t *badj);
+
+/* ------------------------------------------------------------------------------
+ * 4. Allocation-free step.  No counterpart in the reference: for callers that cannot
+ *    allocate device memory (code under HIP graph capture, tight serving loops).
+ * ---------------------------------------------------------------------------- */
+/* Largest grid (nx * ny) of chimera_step_noalloc. */
+#define CHIMERA_NOALLOC_MAX_CELLS (512 * 512)
+/* state_out = one SSP-RK3 step of state_in with the given dt: the bits of
+ * chimera_adj_forward_step (and of chimera_ctx_steps with that dt).  state_in /
+ * state_out: 6 device arrays of nx * ny (Mass, Momx, Momy, Energy, b_x, b_y);
+ * state_out may be state_in.  Asynchronous on `stream` (NULL: the null stream).  Calls
+ * no hipMalloc / hipFree: the two intermediate stage states and the device dt live in
+ * a static device array of the library (one per GPU, about 25 MB, held until the
+ * process ends), so grids above CHIMERA_NOALLOC_MAX_CELLS cells are rejected with
+ * CHIMERA_EINVAL, and calls that overlap on one GPU (other streams or threads) race
+ * on that array. */
+int chimera_step_noalloc(const double *const *state_in, double dt, double *const *state_out,
+                         int nx, int ny, double dx, double gamma, const chimera_bc_t *bc,
+                         const chimera_scheme_t *scheme, void *stream);
 
 #ifdef __cplusplus
 }
