@@ -8,6 +8,8 @@ from __future__ import annotations
 import ctypes as C
 from pathlib import Path
 
+import numpy as np
+
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libchimera_hip.so"
 
@@ -45,6 +47,32 @@ class Scheme(C.Structure):
 
 class ChimeraError(RuntimeError):
     pass
+
+
+def state_ptrs(x, shape=None, what: str = "state"):
+    """The 6 field pointers of a (6, ny, nx) FP64 state, after checking it: a
+    C-contiguous numpy array, or a contiguous CUDA tensor on the current device.  The
+    library copies or writes nx * ny doubles per field, so a wrong shape, dtype or
+    layout is rejected here (ValueError / TypeError) before any copy."""
+    if isinstance(x, np.ndarray):
+        if x.dtype != np.float64 or not x.flags.c_contiguous:
+            raise ValueError(f"{what}: need a C-contiguous float64 array (got {x.dtype})")
+        ptrs = [x[q].ctypes.data for q in range(x.shape[0])] if x.ndim == 3 else None
+    else:
+        import torch
+        if not isinstance(x, torch.Tensor):
+            raise TypeError(f"{what}: need a numpy array or a torch tensor, got {type(x).__name__}")
+        if not x.is_cuda or x.dtype != torch.float64 or not x.is_contiguous():
+            raise ValueError(f"{what}: need a contiguous float64 CUDA tensor "
+                             f"(got {x.dtype} on {x.device}, contiguous={x.is_contiguous()})")
+        if x.device.index != torch.cuda.current_device():
+            raise ValueError(f"{what}: tensor on {x.device}, current device is "
+                             f"cuda:{torch.cuda.current_device()}")
+        ptrs = [x[q].data_ptr() for q in range(x.shape[0])] if x.ndim == 3 else None
+    if x.ndim != 3 or x.shape[0] != 6 or (shape is not None and tuple(x.shape) != tuple(shape)):
+        want = tuple(shape) if shape is not None else "(6, ny, nx)"
+        raise ValueError(f"{what}: shape {tuple(x.shape)}, need {want}")
+    return (C.c_void_p * 6)(*ptrs)
 
 
 _lib = None

@@ -18,13 +18,15 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import BC, Scheme, check
+from ._lib import BC, Scheme, check, state_ptrs
 from .solver import DT_CFL, DT_PRESCRIBED, Grid
 
 
-def _ptrs(t: torch.Tensor):
-    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.shape[0] == 6
-    return (C.c_void_p * 6)(*[t[q].data_ptr() for q in range(6)])
+def _ptrs(t: torch.Tensor, shape=None, what: str = "state"):
+    """Field pointers of a contiguous float64 CUDA state (6, ny, nx), checked."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what}: need a CUDA tensor, got {type(t).__name__}")
+    return state_ptrs(t, shape, what)
 
 
 class AdjointStepper:
@@ -34,6 +36,7 @@ class AdjointStepper:
         L = _lib.lib()
         self._L = L
         self.nx, self.ny = int(nx), int(ny)
+        self.shape = (6, self.ny, self.nx)
         h = C.c_void_p()
         check(L.chimera_adj_create(C.byref(h), self.nx, self.ny, float(dx), float(gamma),
                                    C.byref(BC.of(bc)), C.byref(Scheme.of(scheme)), None),
@@ -53,15 +56,17 @@ class AdjointStepper:
     def forward_step(self, x: torch.Tensor, dt: float, out: torch.Tensor | None = None):
         out = torch.empty_like(x) if out is None else out
         torch.cuda.synchronize()
-        check(self._L.chimera_adj_forward_step(self._h, _ptrs(x), float(dt), _ptrs(out)),
+        check(self._L.chimera_adj_forward_step(self._h, _ptrs(x, self.shape, "x"), float(dt),
+                                               _ptrs(out, self.shape, "out")),
               "chimera_adj_forward_step")
         return out
 
     def vjp(self, x: torch.Tensor, dt: float, lam_out: torch.Tensor, out=None):
         out = torch.empty_like(x) if out is None else out
         torch.cuda.synchronize()
-        check(self._L.chimera_adj_step_vjp(self._h, _ptrs(x), float(dt), _ptrs(lam_out.contiguous()),
-                                           _ptrs(out)), "chimera_adj_step_vjp")
+        check(self._L.chimera_adj_step_vjp(self._h, _ptrs(x, self.shape, "x"), float(dt),
+                                           _ptrs(lam_out.contiguous(), self.shape, "lam_out"),
+                                           _ptrs(out, self.shape, "out")), "chimera_adj_step_vjp")
         return out
 
     def segment_vjp(self, x0: torch.Tensor, dts, lam: torch.Tensor) -> torch.Tensor:
@@ -71,8 +76,10 @@ class AdjointStepper:
         dts = np.ascontiguousarray(dts, dtype=np.float64)
         assert lam.is_contiguous()
         torch.cuda.synchronize()
-        check(self._L.chimera_adj_segment_vjp(self._h, _ptrs(x0), dts.ctypes.data_as(C.POINTER(C.c_double)),
-                                              len(dts), _ptrs(lam)), "chimera_adj_segment_vjp")
+        check(self._L.chimera_adj_segment_vjp(self._h, _ptrs(x0, self.shape, "x0"),
+                                              dts.ctypes.data_as(C.POINTER(C.c_double)),
+                                              len(dts), _ptrs(lam, self.shape, "lam")),
+              "chimera_adj_segment_vjp")
         return lam
 
     def sweep(self, ckpts, dts, seg_steps: int, lam: torch.Tensor) -> torch.Tensor:
@@ -84,9 +91,13 @@ class AdjointStepper:
         nseg = -(-len(dts) // seg_steps)
         if len(ckpts) != nseg:
             raise ValueError(f"{len(ckpts)} checkpoints for {nseg} segments")
-        check(self._L.chimera_adj_sweep(self._h, _ptrs_many(ckpts),
-                                        dts.ctypes.data_as(C.POINTER(C.c_double)), len(dts),
-                                        int(seg_steps), _ptrs(lam)), "chimera_adj_sweep")
+        ck = _ptrs_many(ckpts, [self.shape[1:]], "checkpoint")
+        lp = _ptrs(lam, self.shape, "lam")
+        # the sweep runs on the library's streams: torch's pending work on lam and the
+        # checkpoints (e.g. lam cloned from the incoming gradient) must be done first
+        torch.cuda.synchronize()
+        check(self._L.chimera_adj_sweep(self._h, ck, dts.ctypes.data_as(C.POINTER(C.c_double)),
+                                        len(dts), int(seg_steps), lp), "chimera_adj_sweep")
         return lam
 
     def record(self, x0: torch.Tensor, dts, seg_steps: int, nseg: int,
@@ -96,8 +107,10 @@ class AdjointStepper:
         last segment; that sweep (with x0 as its last checkpoint and the same dts) then
         skips the segment's recompute."""
         dts = np.ascontiguousarray(dts, dtype=np.float64)
-        check(self._L.chimera_adj_record(self._h, _ptrs(x0), dts.ctypes.data_as(C.POINTER(C.c_double)),
-                                         len(dts), int(seg_steps), int(nseg), _ptrs(out)),
+        x0p, outp = _ptrs(x0, self.shape, "x0"), _ptrs(out, self.shape, "out")
+        torch.cuda.synchronize()   # x0 may still be written by torch's stream
+        check(self._L.chimera_adj_record(self._h, x0p, dts.ctypes.data_as(C.POINTER(C.c_double)),
+                                         len(dts), int(seg_steps), int(nseg), outp),
               "chimera_adj_record")
         return out
 
@@ -194,10 +207,17 @@ def evolve(U0: torch.Tensor, dts, prob_or_geom, checkpoint_every: int = 50) -> t
 
 # ---- batched ensembles ---------------------------------------------------------
 
-def _ptrs_many(ts) -> "C.Array":
-    for t in ts:
-        assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.shape[0] == 6
-    return (C.c_void_p * (6 * len(ts)))(*[t[q].data_ptr() for t in ts for q in range(6)])
+def _ptrs_many(ts, shapes=None, what: str = "state") -> "C.Array":
+    """Field pointers of several states (member m's 6 fields at 6 m .. 6 m + 5), each
+    checked; shapes (optional) gives (ny, nx) per state, repeated over the list."""
+    ts = list(ts)
+    out = (C.c_void_p * (6 * len(ts)))()
+    for k, t in enumerate(ts):
+        shape = None if shapes is None else (6,) + tuple(shapes[k % len(shapes)])
+        p = _ptrs(t, shape, f"{what} {k}")
+        for q in range(6):
+            out[6 * k + q] = p[q]
+    return out
 
 
 def _dt_table(dts, members: int) -> np.ndarray:
@@ -233,18 +253,18 @@ class BatchAdjoint:
         d = _dt_table(dts, self.members)
         out = [torch.empty_like(x) for x in states] if out is None else out
         torch.cuda.synchronize()
-        check(self._L.chimera_badj_forward(self._h, _ptrs_many(states),
+        check(self._L.chimera_badj_forward(self._h, _ptrs_many(states, self.shapes),
                                            d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
-                                           _ptrs_many(out)), "chimera_badj_forward")
+                                           _ptrs_many(out, self.shapes)), "chimera_badj_forward")
         return out
 
     def segment_vjp(self, states0, dts, lams) -> None:
         """lams[m] <- (dU^k / dU^0)^T lams[m] in place for every member, k = len(dts[m])."""
         d = _dt_table(dts, self.members)
         torch.cuda.synchronize()
-        check(self._L.chimera_badj_segment_vjp(self._h, _ptrs_many(states0),
+        check(self._L.chimera_badj_segment_vjp(self._h, _ptrs_many(states0, self.shapes),
                                                d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
-                                               _ptrs_many(lams)), "chimera_badj_segment_vjp")
+                                               _ptrs_many(lams, self.shapes)), "chimera_badj_segment_vjp")
 
     def sweep(self, ckpts, dts, seg_steps: int, lams) -> None:
         """lams[m] <- (dU_N / dU_0)^T lams[m] in place for a run of dts.shape[1] steps
@@ -255,9 +275,9 @@ class BatchAdjoint:
         if len(ckpts) != nseg or any(len(c) != self.members for c in ckpts):
             raise ValueError(f"need {nseg} checkpoints of {self.members} members")
         torch.cuda.synchronize()
-        check(self._L.chimera_badj_sweep(self._h, _ptrs_many([t for c in ckpts for t in c]),
+        check(self._L.chimera_badj_sweep(self._h, _ptrs_many([t for c in ckpts for t in c], self.shapes),
                                          d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
-                                         int(seg_steps), _ptrs_many(lams)), "chimera_badj_sweep")
+                                         int(seg_steps), _ptrs_many(lams, self.shapes)), "chimera_badj_sweep")
 
     def record(self, states0, dts, seg_steps: int, nseg: int, out=None):
         """dts.shape[1] (<= seg_steps) steps of every member from states0 into out (new
@@ -267,9 +287,9 @@ class BatchAdjoint:
         d = _dt_table(dts, self.members)
         out = [torch.empty_like(x) for x in states0] if out is None else out
         torch.cuda.synchronize()
-        check(self._L.chimera_badj_record(self._h, _ptrs_many(states0),
+        check(self._L.chimera_badj_record(self._h, _ptrs_many(states0, self.shapes),
                                           d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
-                                          int(seg_steps), int(nseg), _ptrs_many(out)),
+                                          int(seg_steps), int(nseg), _ptrs_many(out, self.shapes)),
               "chimera_badj_record")
         return out
 
@@ -288,7 +308,7 @@ class BatchAdjoint:
         best = (C.c_int * 3)()
         ms = (C.c_double * 6)()
         torch.cuda.synchronize()
-        check(self._L.chimera_badj_autotune(self._h, _ptrs_many(states),
+        check(self._L.chimera_badj_autotune(self._h, _ptrs_many(states, self.shapes),
                                             d.ctypes.data_as(C.POINTER(C.c_double)), int(reps),
                                             best, ms), "chimera_badj_autotune")
         return tuple(best), list(ms)

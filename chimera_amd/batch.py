@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import BC, Scheme, check
+from ._lib import BC, Scheme, check, state_ptrs
 
 DT_REFERENCE, DT_CFL = 0, 1
 SHAPES = {0: (32, 8), 1: (16, 16), 2: (64, 4), 3: (32, 32), 4: (32, 16)}  # 3: not HLLD
@@ -46,17 +46,16 @@ class Ensemble:
 
     def upload(self, member: int, state) -> None:
         """state: numpy (6, ny, nx) float64 or a CUDA float64 tensor of that shape."""
+        shape = (6,) + self.shapes[member]
         if isinstance(state, np.ndarray):
             st = np.ascontiguousarray(state, dtype=np.float64)
-            assert st.shape == (6,) + self.shapes[member]
-            ptrs = (C.c_void_p * 6)(*[st[q].ctypes.data for q in range(6)])
+            ptrs = state_ptrs(st, shape, f"member {member}")
         else:
             import torch
             st = state.contiguous()
-            assert tuple(st.shape) == (6,) + self.shapes[member] and str(st.dtype) == "torch.float64"
+            ptrs = state_ptrs(st, shape, f"member {member}")
             # the copy runs on the library's stream: finish torch's work on the tensor first
             torch.cuda.synchronize()
-            ptrs = (C.c_void_p * 6)(*[st[q].data_ptr() for q in range(6)])
         check(self._L.chimera_batch_upload(self._h, int(member), ptrs), "chimera_batch_upload")
 
     def download(self, member: int) -> np.ndarray:

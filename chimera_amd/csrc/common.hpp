@@ -41,9 +41,9 @@ void set_error(const char *fmt, ...);
 inline bool bc_flag_ok(int f) { return f >= 1 && f <= 4; }
 
 // Per-axis periodicity.  Only the periodic / non-periodic distinction changes the
-// stage update (fixed and inflow fall back to zero-gradient without an ambient
-// state, src/mhd_bc.f90:131-136); a mixed periodic / non-periodic pair on one axis
-// is rejected because the halo of a wrapped tile would be ambiguous.
+// stage update (fixed and inflow fall back to zero-gradient without an ambient state,
+// src/mhd_bc.f90:62-67, 76-81, 92-97, 106-111); a mixed periodic / non-periodic pair
+// on one axis is rejected because the halo of a wrapped tile would be ambiguous.
 int check_bc(const chimera_bc_t *bc, int *px, int *py);
 int check_scheme(const chimera_scheme_t *s);
 

@@ -26,7 +26,7 @@ inline int blocks_for(size_t n) {
 
 __device__ __forceinline__ bool is_fixed_like(int f) { return f == 2 || f == 4; }
 
-// fill_ghost_cells / fill_corners / corner_val, src/mhd_bc.f90:105-239
+// fill_ghost_cells / fill_corners / corner_val, src/mhd_bc.f90:36-170
 __global__ void k_fill_ghost(const double *f, double *pad, const double *amb, int nx, int ny,
                              int xlo, int xhi, int ylo, int yhi) {
     const int W = nx + 2;

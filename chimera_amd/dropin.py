@@ -25,7 +25,7 @@ def _f(a) -> np.ndarray:
 
 
 def fill_ghost_cells(f, bc, f_ambient=None) -> np.ndarray:
-    """src/mhd_bc.f90:105-190 -> (ny+2, nx+2) padded field."""
+    """src/mhd_bc.f90:36-141 -> (ny+2, nx+2) padded field."""
     f = _f(f)
     ny, nx = f.shape
     pad = np.empty((ny + 2, nx + 2))

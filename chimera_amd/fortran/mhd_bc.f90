@@ -10,7 +10,7 @@ module mhd_bc
 
 contains
 
-    ! src/mhd_bc.f90:105-190 -- (N+2) x (N+2) pad of an N x N field (module N).
+    ! src/mhd_bc.f90:36-141 -- (N+2) x (N+2) pad of an N x N field (module N).
     subroutine fill_ghost_cells(f, f_pad, f_ambient)
         real(8), intent(in)  :: f(N, N)
         real(8), intent(out) :: f_pad(N+2, N+2)

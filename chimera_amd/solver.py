@@ -12,17 +12,12 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import BC, Scheme, check
+from ._lib import BC, Scheme, check, state_ptrs
 
 DT_REFERENCE, DT_CFL, DT_PRESCRIBED = 0, 1, 2
 # tile shapes of the fused stage kernel (CHIMERA_CTX_NUM_SHAPES), in shape-index order
 TILE_SHAPES = ((32, 8), (16, 16), (64, 4), (32, 32), (32, 16))
 NUM_TILE_SHAPES = len(TILE_SHAPES)
-
-
-def _host_ptrs(arr: np.ndarray):
-    assert arr.dtype == np.float64 and arr.flags.c_contiguous and arr.shape[0] == 6
-    return (C.c_void_p * 6)(*[arr[q].ctypes.data for q in range(6)])
 
 
 class Grid:
@@ -48,37 +43,40 @@ class Grid:
     # ---- state transfer -------------------------------------------------------
     def upload(self, state) -> None:
         """state: numpy (6, ny, nx) float64, or a CUDA float64 tensor of that shape."""
+        shape = (6, self.ny, self.nx)
         if isinstance(state, np.ndarray):
             st = np.ascontiguousarray(state, dtype=np.float64)
-            assert st.shape == (6, self.ny, self.nx), st.shape
-            check(self._L.chimera_ctx_upload(self._h, _host_ptrs(st)), "chimera_ctx_upload")
+            check(self._L.chimera_ctx_upload(self._h, state_ptrs(st, shape, "upload")),
+                  "chimera_ctx_upload")
         else:
             import torch
             t = state.contiguous()
-            assert tuple(t.shape) == (6, self.ny, self.nx) and str(t.dtype) == "torch.float64"
+            ptrs = state_ptrs(t, shape, "upload")
             # the copy runs on the library's stream: finish torch's work on the tensor first
             torch.cuda.synchronize()
-            ptrs = (C.c_void_p * 6)(*[t[q].data_ptr() for q in range(6)])
             check(self._L.chimera_ctx_upload(self._h, ptrs), "chimera_ctx_upload")
 
     def download(self, out=None):
+        """Into out (numpy or CUDA tensor of shape (6, ny, nx), float64, contiguous; a
+        new numpy array when None)."""
         if out is None:
             out = np.empty((6, self.ny, self.nx))
-        if isinstance(out, np.ndarray):
-            check(self._L.chimera_ctx_download(self._h, _host_ptrs(out)), "chimera_ctx_download")
-        else:
+        ptrs = state_ptrs(out, (6, self.ny, self.nx), "download")
+        if not isinstance(out, np.ndarray):
             import torch
             torch.cuda.synchronize()   # torch may still be using the destination
-            ptrs = (C.c_void_p * 6)(*[out[q].data_ptr() for q in range(6)])
-            check(self._L.chimera_ctx_download(self._h, ptrs), "chimera_ctx_download")
+        check(self._L.chimera_ctx_download(self._h, ptrs), "chimera_ctx_download")
         return out
 
     # ---- stepping ---------------------------------------------------------------
     def steps(self, n: int, dt_mode: int = DT_REFERENCE, dt=None, graph: bool = False) -> None:
         dti = None
         if dt_mode == DT_PRESCRIBED:
+            if dt is None:
+                raise ValueError("DT_PRESCRIBED needs a dt sequence")
             dt = np.ascontiguousarray(dt, dtype=np.float64)
-            assert dt.shape[0] >= n
+            if dt.ndim != 1 or dt.shape[0] < n:
+                raise ValueError(f"DT_PRESCRIBED needs {n} dts, got shape {dt.shape}")
             dti = dt.ctypes.data_as(C.POINTER(C.c_double))
         check(self._L.chimera_ctx_steps(self._h, int(n), int(dt_mode), dti, int(bool(graph))),
               "chimera_ctx_steps")

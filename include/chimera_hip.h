@@ -6,9 +6,9 @@
  * C [ny][nx]: element (i, j) at i + nx*j.  Boundary flags use the reference's
  * constants (src/mhd_config.f90:40-43): 1 periodic, 2 fixed, 3 outflow, 4 inflow;
  * the main loop never supplies an ambient state, so 2/3/4 all act as zero-gradient
- * (src/mhd_bc.f90:128-139, src/main.f90:349-354).  Every function returns 0 on
- * success and a negative chimera_status on failure; chimera_last_error() describes
- * the last failure on the calling thread.
+ * (src/mhd_bc.f90:62-67, 76-81, 92-97, 106-111, src/main.f90:349-354).  Every
+ * function returns 0 on success and a negative chimera_status on failure;
+ * chimera_last_error() describes the last failure on the calling thread.
  *
  * Layers:
  *   1. Drop-ins: one function per reference routine, same argument meaning and order
@@ -67,7 +67,7 @@ int chimera_device_count(int *count);
  * 1. Drop-ins (host pointers).  Each replaces the Fortran routine named.
  * ---------------------------------------------------------------------------- */
 
-/* mhd_bc::fill_ghost_cells, src/mhd_bc.f90:105-190.  f_pad is (nx+2) x (ny+2);
+/* mhd_bc::fill_ghost_cells, src/mhd_bc.f90:36-141.  f_pad is (nx+2) x (ny+2);
  * f_ambient may be NULL (zero-gradient for fixed/inflow, as main.f90 uses it).
  * Unlike the reference this also accepts nx != ny. */
 int chimera_fill_ghost_cells(const double *f, double *f_pad, const double *f_ambient, int nx,
@@ -196,8 +196,11 @@ int chimera_ctx_stage1_mask(chimera_ctx_t *ctx, uint32_t *mask_host);
 int chimera_ctx_sync(chimera_ctx_t *ctx);
 /* The stream the context launches on (hipStream_t). */
 void *chimera_ctx_stream(chimera_ctx_t *ctx);
-/* Average duration (ms) of the fused stage kernel over the last chimera_ctx_steps
- * call, measured with HIP events on the context's stream (0 if not measured). */
+/* Timing of the fused stage kernel: after chimera_ctx_set_timing(ctx, 1) every stage
+ * launch of chimera_ctx_steps with use_graph = 0 is bracketed by HIP events on the
+ * context's stream; chimera_ctx_stage_kernel_ms returns the average duration (ms) and
+ * the number of launches since set_timing (graph replays are not timed: launches stays
+ * 0 and ms 0 when only graph steps ran). */
 int chimera_ctx_set_timing(chimera_ctx_t *ctx, int enable);
 int chimera_ctx_stage_kernel_ms(chimera_ctx_t *ctx, double *ms, int *launches);
 /* Tile shape of the fused stage kernel: 0 = 32x8, 1 = 16x16, 2 = 64x4 cells (256
@@ -242,7 +245,9 @@ int chimera_batch_get_tiling(chimera_batch_t *batch, int *shape, int *blocks_per
 /* Time every (tile shape, blocks per CU) candidate on one stage, keep the fastest. */
 int chimera_batch_autotune(chimera_batch_t *batch, int reps, int *best_shape,
                            int *best_blocks_per_cu);
-/* Average fused-stage launch duration (ms) when timing is enabled (HIP events). */
+/* Average duration (ms) and count of the fused-stage launches of chimera_batch_steps
+ * with use_graph = 0 since chimera_batch_set_timing(batch, 1) (HIP events on the
+ * batch's stream; graph replays are not timed). */
 int chimera_batch_set_timing(chimera_batch_t *batch, int enable);
 int chimera_batch_stage_kernel_ms(chimera_batch_t *batch, double *ms, int *launches);
 

@@ -47,7 +47,7 @@ int orc_num_threads(void) {
 
 /* ---------------------------------------------------------------- mhd_bc */
 
-/* corner_val, src/mhd_bc.f90:213-239 (opposite corner index uses nx / ny). */
+/* corner_val, src/mhd_bc.f90:144-170 (opposite corner index uses nx / ny). */
 static double corner_val(int nx, int ny, const double *f, int ix, int iy, int bcx, int bcy) {
     int ixw = nx - 1 - ix, iyw = ny - 1 - iy;
     if (periodic(bcx) && periodic(bcy)) return f[IX(ixw, iyw)];
@@ -56,7 +56,7 @@ static double corner_val(int nx, int ny, const double *f, int ix, int iy, int bc
     return f[IX(ix, iy)];
 }
 
-/* fill_ghost_cells + fill_corners, src/mhd_bc.f90:105-210.  The reference sizes
+/* fill_ghost_cells + fill_corners, src/mhd_bc.f90:36-141.  The reference sizes
  * everything by the module N (square grids only); the same mapping is applied
  * per axis here so nx != ny also works. */
 void orc_fill_ghost_cells(int nx, int ny, const int *bc, const double *f, double *f_pad,

@@ -14,7 +14,7 @@ Reference: src/main.f90:65-303 (step), src/main.f90:346-403 (stage fluxes),
 src/mhd_derivatives.f90:50-293 (limiter, reconstruction + MOOD, thermal check),
 src/mhd_flux.f90:20-779 (update, CT, Rusanov / HLLE / HLLD),
 src/mhd_change_states.f90:61-166 (primitives, floors), src/mhd_field_ops.f90 (curl,
-face -> cell B), src/mhd_bc.f90:105-210 (ghost cells: wrap or copy).
+face -> cell B), src/mhd_bc.f90:36-141 (ghost cells: wrap or copy).
 """
 from __future__ import annotations
 

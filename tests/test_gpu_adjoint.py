@@ -561,3 +561,57 @@ def test_sweep_reuses_recorded_last_segment(gpu_lib, nsteps, K):
     adj.record(ckpts[-1], dts[s_last:] * 1.01, K, nseg, torch.empty_like(x))
     assert np.array_equal(adj.sweep(ckpts, dts, K, lam0.clone()).cpu().numpy(), ref)
     adj.close()
+
+
+def test_evolve_backward_on_side_stream_matches_default_stream(gpu_lib):
+    """evolve + backward issued under a non-default torch stream give the gradient bits
+    of the default stream (the library's streams wait for torch's work on the inputs)."""
+    torch = _torch()
+    from chimera_amd.adjoint import cell_b2, evolve, field_loop_state, record_dt
+    p, Az = problems.field_loop(96, 48, scheme=(2, 2, 1, 1), return_az=True)
+    dts = record_dt(p, 24)
+    T = torch.from_numpy(np.random.default_rng(3).uniform(0.5, 1.5, Az.shape) * 1e-6).cuda()
+
+    def grad():
+        Azg = torch.from_numpy(Az).cuda().requires_grad_(True)
+        UN = evolve(field_loop_state(Azg, p.dx), dts, p, checkpoint_every=10)
+        ((cell_b2(UN) - T) ** 2).sum().backward()
+        return Azg.grad.clone()
+
+    ref = grad()
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        got = grad()
+    s.synchronize()
+    assert torch.equal(got, ref)
+
+
+def test_bindings_reject_wrong_buffers(gpu_lib):
+    """Grid.download / upload and the adjoint calls refuse a buffer of the wrong shape,
+    dtype or device before any copy."""
+    torch = _torch()
+    from chimera_amd.adjoint import AdjointStepper
+    from chimera_amd.solver import Grid
+    p = problems.orszag_tang(32)
+    with Grid.from_problem(p) as g:
+        with pytest.raises(ValueError):
+            g.download(np.empty((6, 32, 31)))
+        with pytest.raises(ValueError):
+            g.download(np.empty((6, 32, 32), dtype=np.float32))
+        with pytest.raises(ValueError):
+            g.download(torch.empty((6, 32, 32), dtype=torch.float32, device="cuda"))
+        with pytest.raises(ValueError):
+            g.upload(np.zeros((6, 31, 32)))
+        out = torch.empty((6, 32, 32), dtype=torch.float64, device="cuda")
+        g.download(out)
+        assert torch.equal(out.cpu(), torch.from_numpy(p.state))
+    adj = AdjointStepper(p.nx, p.ny, p.dx, p.gamma, p.bc, p.scheme)
+    try:
+        x = torch.from_numpy(p.state).cuda()
+        with pytest.raises(ValueError):
+            adj.forward_step(x, 1e-3, out=torch.empty((6, 32, 16), dtype=torch.float64, device="cuda"))
+        with pytest.raises(ValueError):
+            adj.forward_step(x[:, :16].contiguous(), 1e-3)
+    finally:
+        adj.close()

@@ -228,14 +228,26 @@ def test_full_size_rotor_one_step(gpu_lib, orc):
     assert np.isfinite(st).all() and (st[0] > 0).all()
 
 
-def test_full_size_periodic_conservation(gpu_lib):
-    """OT at 1024^2: total mass is conserved to round-off and div B stays at round-off."""
+def test_full_size_periodic_conservation(gpu_lib, orc):
+    """OT at 1024^2: total mass is conserved to round-off over 20 graph-replayed steps,
+    and after 5 steps the state and its max|div B| equal the oracle's.  CT keeps div B
+    at round-off of the face differences divided by dx, which grows like 1/dx (4.5e-13
+    here after 5 steps, <= 1e-13 at 128^2 above), so at this size the gate is equality
+    with the oracle's max|div B| (src/mhd_field_ops.f90:60-117), not a fixed bound."""
     from chimera_amd.solver import Grid
     prob = problems.orszag_tang(1024)
     m0 = prob.state[0].sum()
+    print("OT 1024^2: 5 steps vs the oracle, then 20 graph steps", flush=True)
     with Grid.from_problem(prob) as g:
-        g.steps(20, graph=True)
+        g.steps(5)
+        st5 = g.download()
+        mx5, _ = g.divb()
+        g.steps(15, graph=True)
         st = g.download()
-        mx, _ = g.divb()
+    cpu = prob.state.copy()
+    orc.run(cpu, prob.dx, prob.gamma, prob.bc, list(prob.scheme), 5)
+    assert np.array_equal(st5, cpu)
+    d = orc.compute_divB(cpu[4], cpu[5], prob.dx, prob.bc)
+    assert mx5 == np.abs(d).max()
+    assert mx5 <= 1e-12
     assert abs(st[0].sum() - m0) / m0 <= 1e-12
-    assert mx <= 1e-10

@@ -113,3 +113,35 @@ def test_scheme_and_bc_structs():
     assert (s.riemann_solver, s.slope_limiter, s.use_slope_limiting, s.fallback_2_mood) == (3, 1, 1, 1)
     with pytest.raises(ValueError):
         Scheme.of((1, 2, 3))
+
+
+def test_state_pointers_reject_bad_buffers():
+    """The Python binding checks shape, dtype and layout of every state it hands to the
+    library (which copies nx * ny doubles per field) before any copy."""
+    import torch
+    from chimera_amd._lib import state_ptrs
+    good = np.zeros((6, 8, 12))
+    assert len(state_ptrs(good, (6, 8, 12))) == 6
+    with pytest.raises(ValueError, match="shape"):
+        state_ptrs(np.zeros((6, 8, 11)), (6, 8, 12))
+    with pytest.raises(ValueError, match="shape"):
+        state_ptrs(np.zeros((5, 8, 12)))
+    with pytest.raises(ValueError, match="float64"):
+        state_ptrs(np.zeros((6, 8, 12), dtype=np.float32), (6, 8, 12))
+    with pytest.raises(ValueError, match="contiguous"):
+        state_ptrs(np.zeros((6, 12, 8)).transpose(0, 2, 1), (6, 8, 12))
+    with pytest.raises(ValueError, match="CUDA"):
+        state_ptrs(torch.zeros((6, 8, 12), dtype=torch.float64), (6, 8, 12))
+    with pytest.raises(TypeError):
+        state_ptrs([[0.0]], (6, 8, 12))
+
+
+def test_prescribed_steps_need_enough_dts(monkeypatch):
+    """Grid.steps rejects a short dt sequence before calling the library."""
+    from chimera_amd import solver
+    g = solver.Grid.__new__(solver.Grid)   # no device context: the check comes first
+    g._L = None
+    with pytest.raises(ValueError, match="dts"):
+        g.steps(5, dt_mode=solver.DT_PRESCRIBED, dt=np.ones(3))
+    with pytest.raises(ValueError):
+        g.steps(5, dt_mode=solver.DT_PRESCRIBED, dt=None)
