@@ -615,3 +615,143 @@ def test_bindings_reject_wrong_buffers(gpu_lib):
             adj.forward_step(x[:, :16].contiguous(), 1e-3)
     finally:
         adj.close()
+
+
+# ---- config 4 at its own grid: field loop 512 x 256 (HLLE, MC) ---------------------------
+
+def test_field_loop_512x256_gradient_exact_segments(gpu_lib):
+    """The field loop at the benchmark grid (512 x 256, cone Az, HLLE / MC, MUSCL + MOOD):
+    the checkpointed GPU gradient of the |B|^2 misfit with respect to every Az node,
+    12 CFL steps in 3 segments (checkpoint every 5), against autograd through the
+    oracle-pinned torch restatement of the same steps (exact branch-wise derivative)."""
+    torch = _torch()
+    from chimera_amd.adjoint import cell_b2, evolve, field_loop_state, record_dt
+    from tests.adjoint_reference import Geometry, step
+    nsteps = 12
+    p, Az = problems.field_loop(512, 256, scheme=(2, 2, 1, 1), return_az=True)
+    dts = record_dt(p, nsteps)
+    T = np.random.default_rng(21).uniform(0.5, 1.5, Az.shape) * 1e-6
+    print("field loop 512x256, 12 steps, checkpoints every 5", flush=True)
+    Azg = torch.from_numpy(Az).cuda().requires_grad_(True)
+    UN = evolve(field_loop_state(Azg, p.dx), dts, p, checkpoint_every=5)
+    ((cell_b2(UN) - torch.from_numpy(T).cuda()) ** 2).sum().backward()
+    got = Azg.grad.cpu().numpy()
+    Azc = torch.from_numpy(Az.copy()).requires_grad_(True)
+    U = field_loop_state(Azc, p.dx)
+    G = Geometry.of(p)
+    for k in range(nsteps):
+        U = step(U, float(dts[k]), G)
+    ((cell_b2(U) - torch.from_numpy(T)) ** 2).sum().backward()
+    ref = Azc.grad.numpy()
+    assert np.abs(ref).max() > 0
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"max |grad - exact| / max |exact| = {err:.2e}", flush=True)
+    assert err <= 1e-11, err
+
+
+def _loop_fd_setup(nx, ny, solver):
+    """Smooth loop (Gaussian Az + a smooth periodic field) on a smooth background, Van
+    Leer, on an nx x ny grid of the field-loop domain's dx = 2 / nx."""
+    from tests.adjoint_cases import smooth_loop_az
+    Az, dx = smooth_loop_az(nx, ny, seed=1)
+    hyd = problems.field_loop_background(nx, ny, seed=20)
+    p = problems.field_loop_from_az(Az, dx, scheme=(solver, 1, 1, 1), hydro=hyd)
+    return Az, dx, hyd, p
+
+
+def _loop_fd_check(run_cpu, Az, dx, hyd, p, nsteps, nodes, torch, kink_guard=False):
+    """Worst relative error of the adjoint against central differences at the `nodes`
+    largest-gradient Az nodes.  kink_guard: a node counts only where central differences
+    converge (step eps and eps / 4 agree to 1e-6), i.e. where no limiter switch lies
+    within eps of the state; up to 3 x nodes candidates are tried and at least `nodes`
+    must qualify."""
+    from chimera_amd.adjoint import cell_b2, evolve, field_loop_state, record_dt
+    dts = record_dt(p, nsteps)
+    scheme = p.scheme
+
+    def final_b2(state):
+        st = state.copy()
+        run_cpu(st, dts)
+        Bx = 0.5 * (st[4] + np.roll(st[4], 1, axis=1))
+        By = 0.5 * (st[5] + np.roll(st[5], 1, axis=0))
+        return Bx * Bx + By * By
+
+    rng = np.random.default_rng(7)
+    T = final_b2(problems.field_loop_from_az(Az * (1.0 + 0.01 * rng.standard_normal(Az.shape)), dx,
+                                             scheme=scheme, hydro=hyd).state)
+
+    def loss_cpu(az):
+        st = problems.field_loop_from_az(az, dx, scheme=scheme, hydro=hyd).state
+        b2 = final_b2(st)
+        return b2, T
+
+    Azt = torch.from_numpy(Az).cuda().requires_grad_(True)
+    UN = evolve(field_loop_state(Azt, dx, hydro=[torch.from_numpy(a).cuda() for a in hyd]), dts, p,
+                checkpoint_every=8)
+    ((cell_b2(UN) - torch.from_numpy(T).cuda()) ** 2).sum().backward()
+    grad = Azt.grad.cpu().numpy()
+    eps = 1e-7 * np.abs(Az).max()
+
+    def fd_at(j, i, e):
+        ap, am = Az.copy(), Az.copy()
+        ap[j, i] += e
+        am[j, i] -= e
+        bp, _ = loss_cpu(ap)
+        bm, _ = loss_cpu(am)
+        # L(ap) - L(am) per cell before the sum: (b+ - b-) (b+ + b- - 2 T)
+        return float(((bp - bm) * (bp + bm - 2.0 * T)).sum()) / (2 * e)
+
+    worst, used = 0.0, 0
+    for idx in np.argsort(-np.abs(grad).ravel())[:(3 * nodes if kink_guard else nodes)]:
+        j, i = divmod(int(idx), Az.shape[1])
+        fd = fd_at(j, i, eps)
+        if kink_guard:
+            fd4 = fd_at(j, i, eps / 4)
+            if abs(fd - fd4) > 1e-6 * abs(fd4):
+                print(f"node ({j},{i}) skipped: central differences do not converge "
+                      f"({fd:.6e} at eps, {fd4:.6e} at eps/4)", flush=True)
+                continue
+        rel = abs(grad[j, i] - fd) / abs(fd)
+        print(f"node ({j},{i}) adj {grad[j, i]:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
+        worst = max(worst, rel)
+        used += 1
+        if used == nodes:
+            break
+    assert used == nodes, f"only {used} of {nodes} nodes with converging differences"
+    return worst
+
+
+def test_field_loop_512x256_gradient_vs_oracle_fd(gpu_lib, orc):
+    """At the benchmark grid itself (512 x 256, HLLE, Van Leer smooth loop, 20 CFL steps,
+    dt frozen): dL/dAz at the 8 largest-gradient nodes against central differences of
+    the oracle (the C restatement, bit-pinned to the reference; the reference Fortran
+    itself is square-only, module N), relative step 1e-7 of max|Az|."""
+    torch = _torch()
+    Az, dx, hyd, p = _loop_fd_setup(512, 256, 2)
+    print("field loop 512x256 smooth, 20 steps, FD of the oracle", flush=True)
+
+    def run_cpu(st, dts):
+        orc.run(st, dx, p.gamma, p.bc, list(p.scheme), len(dts), dt_mode=2, dt_in=dts)
+
+    worst = _loop_fd_check(run_cpu, Az, dx, hyd, p, 20, 8, torch)
+    assert worst <= 1e-6, worst
+
+
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built")
+def test_field_loop_benchmark_dx_gradient_vs_reference_fd(gpu_lib):
+    """The same check at the benchmark's dx = 1/256 against central differences of the
+    Fortran reference itself, on the largest square window of the field-loop domain it
+    can run (512 x 512 on [-1,1]^2; the reference grid is N x N): 20 CFL steps, HLLE, Van
+    Leer smooth loop, the 6 largest-gradient nodes at which central differences
+    converge (at this resolution neighbouring values of the smooth fields come within
+    1e-9 of each other, so a perturbation can cross a Van Leer switch)."""
+    torch = _torch()
+    Az, dx, hyd, p = _loop_fd_setup(512, 512, 2)
+    ref = RefVariant(2, 1)
+    print("field loop 512x512 smooth, 20 steps, FD of the reference Fortran", flush=True)
+
+    def run_cpu(st, dts):
+        ref.run(st, dx, p.gamma, p.bc, len(dts), dt_mode=2, dt_in=dts)
+
+    worst = _loop_fd_check(run_cpu, Az, dx, hyd, p, 20, 6, torch, kink_guard=True)
+    assert worst <= 1e-6, worst
