@@ -58,9 +58,15 @@ class Ensemble:
             torch.cuda.synchronize()
         check(self._L.chimera_batch_upload(self._h, int(member), ptrs), "chimera_batch_upload")
 
-    def download(self, member: int) -> np.ndarray:
-        out = np.empty((6,) + self.shapes[member])
-        ptrs = (C.c_void_p * 6)(*[out[q].ctypes.data for q in range(6)])
+    def download(self, member: int, out=None):
+        """Member `member`'s state into out (numpy or CUDA float64 (6, ny, nx); a new
+        numpy array when None)."""
+        if out is None:
+            out = np.empty((6,) + self.shapes[member])
+        ptrs = state_ptrs(out, (6,) + self.shapes[member], f"member {member}")
+        if not isinstance(out, np.ndarray):
+            import torch
+            torch.cuda.synchronize()   # torch may still be using the destination
         check(self._L.chimera_batch_download(self._h, int(member), ptrs), "chimera_batch_download")
         return out
 
