@@ -220,6 +220,15 @@ static void batch_drop_graphs(chimera_batch *b) {
             }
 }
 
+// Blocks of the persistent stage launch.  Default: 16 x the blocks a CU holds at once.
+// The launch then has more blocks than fit: blocks start whenever a slot frees up, so
+// the blocks sharing a CU run out of phase (one block's halo loads and barriers overlap
+// the other's arithmetic) instead of marching through the phases of every tile in step,
+// and each block walks fewer tiles, which shortens the launch's tail (measured on the
+// 512-member GRF ensemble, 32 x 16 tiles: 4.65 ms per stage with 2 blocks per CU, 4.34
+// with 16, 4.26 with 32 or 64).
+constexpr int BATCH_OVERSUBSCRIBE = 16;
+
 static int batch_set_grid(chimera_batch *b) {
     int bpc = b->blocks_per_cu;
     if (bpc <= 0) {
@@ -227,10 +236,12 @@ static int batch_set_grid(chimera_batch *b) {
         BStageFn fn = bstage_fn(b->sch.riemann_solver, b->sch.slope_limiter, b->shape);
         CH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn,
                                                             shape_threads(b->shape), 0));
-        bpc = occ > 0 ? occ : 1;
+        bpc = (occ > 0 ? occ : 1) * BATCH_OVERSUBSCRIBE;
     }
-    int g = b->cu_count * bpc;
-    g = std::max(8, g / 8 * 8);
+    long long g = (long long)b->cu_count * bpc;
+    const long long tiles8 = ((long long)b->ntiles[b->shape] + 7) / 8 * 8;   // no idle blocks
+    g = std::min(g, std::max(8ll, tiles8));
+    g = std::max(8ll, g / 8 * 8);
     b->grid = g;
     batch_drop_graphs(b);
     return CHIMERA_OK;
@@ -489,7 +500,7 @@ extern "C" int chimera_batch_set_tiling(chimera_batch_t *b, int shape, int block
     CH_REQUIRE(bstage_fn(b->sch.riemann_solver, b->sch.slope_limiter, shape),
                "tile shape %d is not available for riemann_solver %d", shape,
                b->sch.riemann_solver);
-    CH_REQUIRE(blocks_per_cu >= 0 && blocks_per_cu <= 16, "blocks_per_cu must be 0..16");
+    CH_REQUIRE(blocks_per_cu >= 0 && blocks_per_cu <= 256, "blocks_per_cu must be 0..256");
     CH_HIP(hipStreamSynchronize(b->stream));
     b->shape = shape;
     b->blocks_per_cu = blocks_per_cu;
@@ -505,8 +516,8 @@ extern "C" int chimera_batch_get_tiling(chimera_batch_t *b, int *shape, int *blo
 
 // Time one stage (the stage-1 form: reads u^n, writes the copy that stage 1 of the
 // next step overwrites anyway; the members' states are not advanced) for every tile
-// shape and persistent
-// blocks-per-CU from 1 to the occupancy limit; keep the fastest.
+// shape and blocks-per-CU count (1 to the occupancy limit, then 2 to 32 times it); keep
+// the fastest.
 extern "C" int chimera_batch_autotune(chimera_batch_t *b, int reps, int *best_shape,
                                       int *best_bpc) {
     CH_REQUIRE(b, "batch is NULL");
@@ -523,7 +534,11 @@ extern "C" int chimera_batch_autotune(chimera_batch_t *b, int reps, int *best_sh
         if (!fn) continue;
         CH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn,
                                                             shape_threads(sh), 0));
-        for (int bpc = 1; bpc <= std::max(1, occ); ++bpc) {
+        // 1 .. occ resident blocks per CU, then oversubscribed grids of 2, 4, .. 32 x occ
+        std::vector<int> cands;
+        for (int bpc = 1; bpc <= std::max(1, occ); ++bpc) cands.push_back(bpc);
+        for (int f = 2; f <= 32; f *= 2) cands.push_back(std::max(1, occ) * f);
+        for (int bpc : cands) {
             b->shape = sh;
             b->blocks_per_cu = bpc;
             CH_TRY(batch_set_grid(b));

@@ -239,10 +239,14 @@ int chimera_batch_sync(chimera_batch_t *batch);
 void *chimera_batch_stream(chimera_batch_t *batch);
 /* Total cells over all members. */
 int chimera_batch_cells(chimera_batch_t *batch, long long *cells);
-/* Tile shape (as chimera_ctx_set_tile_shape) and persistent blocks per CU (0 = auto). */
+/* Tile shape (as chimera_ctx_set_tile_shape) and blocks per CU of the stage launch
+ * (0..256; 0 = 16 x the blocks a CU holds at once).  Every block loops over the tile
+ * table; counts above what a CU holds oversubscribe the launch, which keeps the blocks
+ * sharing a CU out of phase.  Every setting gives the same bits. */
 int chimera_batch_set_tiling(chimera_batch_t *batch, int shape, int blocks_per_cu);
 int chimera_batch_get_tiling(chimera_batch_t *batch, int *shape, int *blocks_per_cu);
-/* Time every (tile shape, blocks per CU) candidate on one stage, keep the fastest. */
+/* Time every (tile shape, blocks per CU) candidate on one stage (1 .. the occupancy
+ * limit, then 2 .. 32 times it), keep the fastest. */
 int chimera_batch_autotune(chimera_batch_t *batch, int reps, int *best_shape,
                            int *best_blocks_per_cu);
 /* Average duration (ms) and count of the fused-stage launches of chimera_batch_steps

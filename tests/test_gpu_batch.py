@@ -20,13 +20,15 @@ def _members():
 
 
 @pytest.mark.parametrize("graph", [False, True])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
-def test_batch_members_match_oracle(gpu_lib, orc, graph, shape):
+@pytest.mark.parametrize("shape,bpc", [(0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (4, 1), (0, 64)])
+def test_batch_members_match_oracle(gpu_lib, orc, graph, shape, bpc):
+    """bpc: blocks per CU of the stage launch (0: the default, 1: one resident block per
+    CU walking many tiles, 64: far more blocks than fit at once)."""
     from chimera_amd.batch import Ensemble, DT_CFL
     probs = _members()
-    print(f"batch graph={graph} shape={shape}", flush=True)
+    print(f"batch graph={graph} shape={shape} bpc={bpc}", flush=True)
     with Ensemble(probs) as ens:
-        ens.set_tiling(shape, 0)
+        ens.set_tiling(shape, bpc)
         ens.steps(4, dt_mode=DT_CFL, graph=graph)
         outs = [ens.download(m) for m in range(len(probs))]
         times = [ens.member_time(m) for m in range(len(probs))]
