@@ -267,10 +267,15 @@ struct chimera_badj {
 
 namespace {
 
-int badj_grid(chimera_badj *b, const void *fn, int threads, int &grid) {
+// Blocks of a persistent pass over a table of ntiles tiles: 16 x what the CUs hold at
+// once (capped at the tile count), as the batched forward ensemble sizes its launch --
+// blocks that start whenever a slot frees up keep the blocks sharing a CU out of phase.
+int badj_grid(chimera_badj *b, const void *fn, int threads, int ntiles, int &grid) {
     int occ = 0;
     CH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, 0));
-    grid = std::max(8, b->cu_count * std::max(occ, 1) / 8 * 8);
+    long long g = (long long)b->cu_count * std::max(occ, 1) * 16;
+    g = std::min(g, ((long long)ntiles + 7) / 8 * 8);
+    grid = (int)std::max(8ll, g / 8 * 8);
     return CHIMERA_OK;
 }
 
@@ -360,7 +365,7 @@ int badj_forward(chimera_badj *b, const BAdjMember *mem, int nsteps, bool pure) 
     const BFwdFn fn = pure ? fwd_fn<true>(b->sch.riemann_solver, b->sch.slope_limiter, b->shape_f)
                            : fwd_fn<false>(b->sch.riemann_solver, b->sch.slope_limiter, b->shape_f);
     int grid;
-    CH_TRY(badj_grid(b, (const void *)fn, STAGE_THREADS, grid));
+    CH_TRY(badj_grid(b, (const void *)fn, STAGE_THREADS, b->ntiles[b->shape_f], grid));
     for (int s = 0; s < nsteps; ++s)
         for (int stage = 1; stage <= 3; ++stage) {
             hipLaunchKernelGGL(fn, dim3(grid), dim3(STAGE_THREADS), 0, b->stream, mem,
@@ -379,8 +384,8 @@ int badj_reverse(chimera_badj *b, const BAdjMember *mem, int nsteps) {
     const int ta = BADJ_SHAPES[b->shape_a][0] * BADJ_SHAPES[b->shape_a][1];
     const int tb = BADJ_SHAPES[b->shape_b][0] * BADJ_SHAPES[b->shape_b][1];
     int ga, gb;
-    CH_TRY(badj_grid(b, (const void *)fa, ta, ga));
-    CH_TRY(badj_grid(b, (const void *)fb, tb, gb));
+    CH_TRY(badj_grid(b, (const void *)fa, ta, b->ntiles[b->shape_a], ga));
+    CH_TRY(badj_grid(b, (const void *)fb, tb, b->ntiles[b->shape_b], gb));
     for (int s = nsteps - 1; s >= 0; --s)
         for (int st = 3; st >= 1; --st) {
             hipLaunchKernelGGL(fa, dim3(ga), dim3(ta), 0, b->stream, mem, b->dtiles[b->shape_a],
