@@ -123,8 +123,7 @@ __device__ __forceinline__ unsigned long long face_pair(const double *b0, int ps
         if (q == F_RHO) fbits |= fb << (2 * dir);
         if (q == F_P) fbits |= fb << (4 + 2 * dir);
     }
-    if (thermal_reset(L, C)) bits |= 1u << (24 + 2 * dir);
-    if (thermal_reset(R, C)) bits |= 1u << (25 + 2 * dir);
+    bits |= thermal_pair(L, R, C) << (24 + 2 * dir);
     return (unsigned long long)bits | ((unsigned long long)fbits << 32);
 }
 
@@ -144,6 +143,8 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
     const double dx = g.dx, vol = g.vol, gam = g.gamma;
     const double inv_dx = 1.0 / dx, inv_2dx = 1.0 / (2.0 * dx), hdx = 0.5 * dx;
     const bool use_lim = G.use_lim != 0, mood = G.mood != 0;
+    const bool vol_p2 = pow2_value(vol);   // the divisions by vol are then exact products
+    const double ivol = 1.0 / vol;
 
     // loads of phase 3 (Up as the forward stage left it, u^n and lam_out of cells
     // [0, TX] x [0, TY]), issued first so that they arrive while phase 1 computes
@@ -220,7 +221,12 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
         const double Bx = 0.5 * (pl[u][4] + pl[u][6]);
         const double By = 0.5 * (pl[u][5] + pl[u][7]);
         double rho, vx, vy, P;
-        cons_to_prim(pl[u][0], pl[u][1], pl[u][2], pl[u][3], Bx, By, gam, vol, rho, vx, vy, P);
+        if (vol_p2)
+            cons_to_prim_v<true>(pl[u][0], pl[u][1], pl[u][2], pl[u][3], Bx, By, gam, vol, ivol, rho,
+                                 vx, vy, P);
+        else
+            cons_to_prim_v<false>(pl[u][0], pl[u][1], pl[u][2], pl[u][3], Bx, By, gam, vol, ivol,
+                                  rho, vx, vy, P);
         sP[0 * A::P_SZ + idx] = rho;
         sP[1 * A::P_SZ + idx] = vx;
         sP[2 * A::P_SZ + idx] = vy;
@@ -293,7 +299,7 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
         for (int q = 0; q < 6; ++q) P6[q] = sP[q * A::P_SZ + pc];
         const double Bx = P6[4], By = P6[5];
         double U3[4] = {Upc[0], Upc[1], Upc[2], Upc[3]};
-        conserved_floors(U3[0], U3[1], U3[2], U3[3], Bx, By, vol, gam);
+        conserved_floors_fast(U3[0], U3[1], U3[2], U3[3], Bx, By, vol, gam);
         if (xlo_open || ylo_open) {
             const double rho = P6[0], vx = P6[1], vy = P6[2], P = P6[3];
             const double h = 0.5 * (Bx * Bx + By * By);

@@ -96,6 +96,29 @@ __device__ __forceinline__ double cell_slope(double m, double c, double p, bool 
     return use_lim ? limited_slope<LIMITER>(m, c, p, inv_dx) : central_slope(m, p, inv_2dx);
 }
 
+// Raw FP64 min / max (v_min_f64 / v_max_f64).  fmin / fmax of values the compiler
+// cannot prove canonical (LDS loads, selects of them) get a canonicalising v_max of
+// each operand first, to quiet signalling NaNs; the stage never holds a signalling
+// NaN, and for every other input these are fmin / fmax exactly (IEEE mode: a quiet
+// NaN operand yields the other one).  The MOOD bounds and face floors of a face use
+// 16-20 of them.
+__device__ __forceinline__ double vmin(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double vmax(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// the same with a wave-uniform second operand (a constant) held in SGPRs
+__device__ __forceinline__ double vmax_s(double a, double s) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "s"(s));
+    return r;
+}
+
 // Face states of one field at one face from the slopes of its two cells: the MUSCL
 // left state (cell c extrapolated forward), the right state (next cell extrapolated
 // backward, or the cell value itself at a non-periodic hi boundary), then MOOD against
@@ -109,17 +132,17 @@ __device__ __forceinline__ unsigned face_states(double m, double c, double p, do
     double r = last ? c : p - hdx * sr;
     unsigned bits = 0;
     if (mood) {
-        double lo = fmin(fmin(fmin(fmin(c, p), m), nA), nB);
-        double hi = fmax(fmax(fmax(fmax(c, p), m), nA), nB);
-        if (rho_or_p) lo = fmax(1e-9, lo);
+        double lo = vmin(vmin(vmin(vmin(c, p), m), nA), nB);
+        const double hi = vmax(vmax(vmax(vmax(c, p), m), nA), nB);
+        if (rho_or_p) lo = vmax_s(lo, 1e-9);
         if (l < lo || l > hi) { l = c; bits |= 1u; }
         if (r < lo || r > hi) { r = c; bits |= 2u; }
     }
     floor_bits = 0u;
     if (rho_or_p) {
         floor_bits = (1.0e-8 > l ? 1u : 0u) | (1.0e-8 > r ? 2u : 0u);
-        l = fmax(l, 1.0e-8);
-        r = fmax(r, 1.0e-8);
+        l = vmax_s(l, 1.0e-8);
+        r = vmax_s(r, 1.0e-8);
     }
     fl = l;
     fr = r;
@@ -209,27 +232,21 @@ __device__ __forceinline__ bool thermal_reset(double f[6], const double c[6]) {
     return false;
 }
 
-// Raw FP64 min / max (v_min_f64 / v_max_f64).  fmin / fmax of values the compiler
-// cannot prove canonical (LDS loads, selects of them) get a canonicalising v_max of
-// each operand first, to quiet signalling NaNs; the stage never holds a signalling
-// NaN, and for every other input these are fmin / fmax exactly (IEEE mode: a quiet
-// NaN operand yields the other one).  The MOOD bounds and face floors of a face use
-// 16-20 of them.
-__device__ __forceinline__ double vmin(double a, double b) {
-    double r;
-    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-__device__ __forceinline__ double vmax(double a, double b) {
-    double r;
-    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-// the same with a wave-uniform second operand (a constant) held in SGPRs
-__device__ __forceinline__ double vmax_s(double a, double s) {
-    double r;
-    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "s"(s));
-    return r;
+// The thermal check of both states of a face (the same values as two thermal_reset
+// calls); the resets are rare, so their selects run only in waves where a face resets.
+// Returns bit 0 for the left state, bit 1 for the right.
+__device__ __forceinline__ unsigned thermal_pair(double L[6], double R[6], const double C[6]) {
+    const double pthL = L[F_P] - 0.5 * (L[F_BX] * L[F_BX] + L[F_BY] * L[F_BY]);
+    const double pthR = R[F_P] - 0.5 * (R[F_BX] * R[F_BX] + R[F_BY] * R[F_BY]);
+    const bool tl = pthL < P_FLOOR, tr = pthR < P_FLOOR;
+    if (__builtin_amdgcn_ballot_w64(tl || tr)) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            if (tl) L[q] = C[q];
+            if (tr) R[q] = C[q];
+        }
+    }
+    return (tl ? 1u : 0u) | (tr ? 2u : 0u);
 }
 
 // Face states of one field at one face without the diagnostic bits: the MUSCL left
@@ -469,17 +486,8 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             // to the cell values below P_floor; rare, so the selects run only in waves
             // where some face resets
             {
-                const double pthL = L[F_P] - 0.5 * (L[F_BX] * L[F_BX] + L[F_BY] * L[F_BY]);
-                const double pthR = R[F_P] - 0.5 * (R[F_BX] * R[F_BX] + R[F_BY] * R[F_BY]);
-                const bool tl = pthL < P_FLOOR, tr = pthR < P_FLOOR;
-                if (BITS) bits |= (tl ? 1u << (24 + 2 * dir) : 0u) | (tr ? 1u << (25 + 2 * dir) : 0u);
-                if (__builtin_amdgcn_ballot_w64(tl || tr)) {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) {
-                        if (tl) L[q] = C[q];
-                        if (tr) R[q] = C[q];
-                    }
-                }
+                const unsigned tb = thermal_pair(L, R, C);
+                if (BITS) bits |= tb << (24 + 2 * dir);
             }
             const int fx = dir ? run.line : run.start + t;
             const int fy = dir ? run.start + t : run.line;
