@@ -251,22 +251,37 @@ __device__ __forceinline__ unsigned thermal_pair(double L[6], double R[6], const
 
 // Face states of one field at one face without the diagnostic bits: the MUSCL left
 // state, the right state, MOOD against the 5-point stencil of the left cell (Q1: the
-// right state too) and the rho/P floor (src/mhd_derivatives.f90:150-229).  At a
-// non-periodic hi boundary the reference takes the cell value itself for the right
-// state; here the clamped halo makes p = pp = c there, so the right cell's slope is
-// exactly 0 (limited: dfL * dfR = 0; central: (c - c) / 2dx) and p - hdx * 0 = c,
-// the same bits without a test.
-__device__ __forceinline__ void face_states_nb(double m, double c, double p, double nA, double nB,
-                                               double sl, double sr, bool mood, bool rho_or_p,
-                                               double hdx, double &fl, double &fr) {
+// right state too) and the rho/P floor (src/mhd_derivatives.f90:150-229).  b points at
+// the left cell c in LDS; p = b[sA] is the next cell along the face normal, the other
+// stencil members are b[-sA], b[sT] and b[-sT].  At a non-periodic hi boundary the
+// reference takes the cell value itself for the right state; here the clamped halo
+// makes p = pp = c there, so the right cell's slope is exactly 0 (limited:
+// dfL * dfR = 0; central: (c - c) / 2dx) and p - hdx * 0 = c, the same bits without a
+// test.
+//
+// MOOD resets a state only outside [lo, hi], the extremes of the stencil, which holds
+// c and p; a state inside [min(c, p), max(c, p)] (and for rho and P at or above the
+// 1e-9 floor of lo) is therefore never reset.  The limited states of smooth data are
+// inside; only where some lane of the wave is not are the other three stencil members
+// read and the full test made (the same result for every lane).
+__device__ __forceinline__ void face_states_nb(const double *b, int sA, int sT, double c,
+                                               double p, double sl, double sr, bool mood,
+                                               bool rho_or_p, double hdx, double &fl,
+                                               double &fr) {
     double l = c + hdx * sl;
     double r = p - hdx * sr;
     if (mood) {
-        double lo = vmin(vmin(vmin(vmin(c, p), m), nA), nB);
-        const double hi = vmax(vmax(vmax(vmax(c, p), m), nA), nB);
-        if (rho_or_p) lo = vmax_s(lo, 1e-9);
-        if (l < lo || l > hi) l = c;
-        if (r < lo || r > hi) r = c;
+        const double mn = vmin(c, p), mx = vmax(c, p);
+        bool inside = l >= mn && l <= mx && r >= mn && r <= mx;
+        if (rho_or_p) inside = inside && l >= 1e-9 && r >= 1e-9;
+        if (__builtin_amdgcn_ballot_w64(!inside)) {
+            const double m = b[-sA], nA = b[sT], nB = b[-sT];
+            double lo = vmin(vmin(vmin(vmin(c, p), m), nA), nB);
+            const double hi = vmax(vmax(vmax(vmax(c, p), m), nA), nB);
+            if (rho_or_p) lo = vmax_s(lo, 1e-9);
+            if (l < lo || l > hi) l = c;
+            if (r < lo || r > hi) r = c;
+        }
     }
     if (rho_or_p) {
         l = vmax_s(l, 1.0e-8);
@@ -458,7 +473,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 const double *b = bf + q * S::P_SZ;
-                const double m = b[-sA], c = b[0], p = b[sA], pp = b[2 * sA];
+                const double c = b[0], p = b[sA], pp = b[2 * sA];
                 C[q] = c;
                 const double sl = sprev[q];
                 double sr;
@@ -472,14 +487,14 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 sprev[q] = sr;
                 if (BITS) {
                     unsigned fb;
-                    bits |= face_states(m, c, p, b[sT], b[-sT], sl, sr, false, mood,
+                    bits |= face_states(b[-sA], c, p, b[sT], b[-sT], sl, sr, false, mood,
                                         q == F_RHO || q == F_P, hdx, L[q], R[q], fb)
                             << (4 * q + 2 * dir);
                     if (q == F_RHO) fbits |= fb << (2 * dir);
                     if (q == F_P) fbits |= fb << (4 + 2 * dir);
                 } else {
-                    face_states_nb(m, c, p, b[sT], b[-sT], sl, sr, mood, q == F_RHO || q == F_P,
-                                   hdx, L[q], R[q]);
+                    face_states_nb(b, sA, sT, c, p, sl, sr, mood, q == F_RHO || q == F_P, hdx,
+                                   L[q], R[q]);
                 }
             }
             // thermal-pressure check (src/mhd_derivatives.f90:262-267): both faces reset
