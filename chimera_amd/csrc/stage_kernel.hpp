@@ -88,6 +88,8 @@ __device__ __forceinline__ int map_index(int g, int n, int periodic) {
 
 // Field order in LDS / face registers: rho, vx, vy, P, Bx, By.
 enum { F_RHO = 0, F_VX = 1, F_VY = 2, F_P = 3, F_BX = 4, F_BY = 5 };
+// The field that holds role q (rho, u, w, P, Bn, Bt) of a y face: vx <-> vy, Bx <-> By.
+__device__ constexpr int kSwapUW[6] = {0, 2, 1, 3, 5, 4};
 
 // Limited (or central) slope of a cell from its two neighbours along one direction.
 template <int LIMITER>
@@ -449,6 +451,15 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         double *cf = dir ? sFY + fy0 * TX + (fx0 - 1) : sFX + (fy0 - 1) * S::FXW + fx0;
         const int cf_step = dir ? TX : 1;
         const int off_n = dir ? 2 * fsz : fsz, off_t = dir ? fsz : 2 * fsz;
+        // The face states are built in the Riemann solvers' roles (rho, u, w, P, Bn, Bt):
+        // u / Bn the normal, w / Bt the tangential components, so y faces read vy for u,
+        // vx for w, By for Bn and Bx for Bt -- an offset of one field in the LDS address
+        // instead of a select of every value.
+        const int so = dir ? S::P_SZ : 0;
+        auto role = [&](const double *bf, int q) -> const double * {
+            return (q == 1 || q == 4) ? bf + q * S::P_SZ + so
+                                      : ((q == 2 || q == 5) ? bf + q * S::P_SZ - so : bf + q * S::P_SZ);
+        };
         // slope of the first face's left cell; afterwards each face's right-cell slope
         // is carried to the next face, and with limiting the right cell's one-sided
         // difference dfR, which is the next right cell's dfL
@@ -456,7 +467,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         if (run.len > 0) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double *b = b0 + q * S::P_SZ;
+                const double *b = role(b0, q);
                 if (use_lim) {
                     const double dR = (b[sA] - b[0]) * inv_dx;
                     sprev[q] = limiter_of<LIMITER>((b[0] - b[-sA]) * inv_dx, dR);
@@ -472,7 +483,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             unsigned bits = 0, fbits = 0;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double *b = bf + q * S::P_SZ;
+                const double *b = role(bf, q);
                 const double c = b[0], p = b[sA], pp = b[2 * sA];
                 C[q] = c;
                 const double sl = sprev[q];
@@ -486,10 +497,12 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 }
                 sprev[q] = sr;
                 if (BITS) {
+                    // bits by field: role q of a y face is field kSwapUW[q]
+                    const int fq = dir ? kSwapUW[q] : q;
                     unsigned fb;
                     bits |= face_states(b[-sA], c, p, b[sT], b[-sT], sl, sr, false, mood,
                                         q == F_RHO || q == F_P, hdx, L[q], R[q], fb)
-                            << (4 * q + 2 * dir);
+                            << (4 * fq + 2 * dir);
                     if (q == F_RHO) fbits |= fb << (2 * dir);
                     if (q == F_P) fbits |= fb << (4 + 2 * dir);
                 } else {
@@ -499,7 +512,8 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             }
             // thermal-pressure check (src/mhd_derivatives.f90:262-267): both faces reset
             // to the cell values below P_floor; rare, so the selects run only in waves
-            // where some face resets
+            // where some face resets.  In roles the magnetic term is Bn^2 + Bt^2, for y
+            // faces By^2 + Bx^2: the same sum (IEEE addition commutes).
             {
                 const unsigned tb = thermal_pair(L, R, C);
                 if (BITS) bits |= tb << (24 + 2 * dir);
@@ -513,8 +527,9 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                     size_t ko = (size_t)(x0 + i) + (size_t)nx * (y0 + j);
 #pragma unroll
                     for (int q = 0; q < 6; ++q) {
-                        store->face[4 * q + 2 * dir][ko] = L[q];
-                        store->face[4 * q + 2 * dir + 1][ko] = R[q];
+                        const int fq = dir ? kSwapUW[q] : q;
+                        store->face[4 * fq + 2 * dir][ko] = L[q];
+                        store->face[4 * fq + 2 * dir + 1][ko] = R[q];
                     }
                     unsigned long long w = (unsigned long long)bits | ((unsigned long long)fbits << 32);
                     if (dir) store->bits_y[ko] = w;
@@ -523,10 +538,8 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 if (MASK && owned && bits)
                     atomicOr(&g.mask[(size_t)(x0 + i) + (size_t)nx * (y0 + j)], bits);
             }
-            FaceState a{L[F_RHO], dir ? L[F_VY] : L[F_VX], dir ? L[F_VX] : L[F_VY], L[F_P],
-                        dir ? L[F_BY] : L[F_BX], dir ? L[F_BX] : L[F_BY]};
-            FaceState bR{R[F_RHO], dir ? R[F_VY] : R[F_VX], dir ? R[F_VX] : R[F_VY], R[F_P],
-                         dir ? R[F_BY] : R[F_BX], dir ? R[F_BX] : R[F_BY]};
+            FaceState a{L[0], L[1], L[2], L[3], L[4], L[5]};
+            FaceState bR{R[0], R[1], R[2], R[3], R[4], R[5]};
             Flux5 f = riemann<SOLVER>(a, bR, gam);
             // transverse flux: By flux (x faces) / Bx flux (y faces), every face
             bt[t * bt_step] = f.bt;
