@@ -91,13 +91,81 @@ __device__ __forceinline__ double cfl_speed(double rho, double vx, double vy, do
     return cf + sqrt(vx * vx + vy * vy);
 }
 
+// ---- FP64 division and square root for operands in a known range ---------------
+// hipcc lowers a / b to d = div_scale(b), n = div_scale(a), y = rcp(d), two Newton
+// steps y += y (1 - d y), q = n y, r = n - d q, q' = div_fmas(r, y, q) and
+// div_fixup(q', b, a).  div_scale rescales only an operand that is zero, infinite, NaN
+// or denormal, a numerator below 2^-969, a reciprocal or quotient in the denormal
+// range or exponents more than 767 apart; otherwise both are the identity, div_fmas
+// is a plain fma and div_fixup returns q' (with the sign of a / b, which q' already
+// has).  sqrt(x) is rsq(x) and the coupled Newton steps below, with a 2^256 pre-scale
+// for x < 2^-767 and a select for zero and infinity.  For finite nonzero a and b with
+// |a|, |b| and |a / b| in [2^-900, 2^700], and for x in [2^-700, 2^700] -- the callers
+// establish this before they use the short forms -- the core steps alone are the
+// whole computation, so these return the bits of a / b and sqrt(x).
+struct Recip {
+    double b, y;   // the divisor and its Newton-refined reciprocal
+};
+__device__ __forceinline__ Recip recip_nr(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    return {b, y};
+}
+__device__ __forceinline__ double div_nr(double a, Recip r) {
+    const double q = a * r.y;
+    const double e = fma(-r.b, q, a);
+    return fma(e, r.y, q);
+}
+__device__ __forceinline__ double sqrt_nr(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, h = y * 0.5;
+    const double e = fma(-h, s, 0.5);
+    s = fma(s, e, s);
+    h = fma(h, e, h);
+    double d = fma(-s, s, x);
+    s = fma(d, h, s);
+    d = fma(-s, s, x);
+    return fma(d, h, s);
+}
+// a / b and sqrt(x): the short forms when the operands are in range (SHORT).
+template <bool SHORT>
+__device__ __forceinline__ double qdiv(double a, double b) {
+    return SHORT ? div_nr(a, recip_nr(b)) : a / b;
+}
+template <bool SHORT>
+__device__ __forceinline__ double qsqrt(double x) {
+    return SHORT ? sqrt_nr(x) : sqrt(x);
+}
+// The reciprocal of a wave-uniform divisor, held in scalar registers.
+__device__ __forceinline__ Recip recip_uniform(double b) {
+    const Recip r = recip_nr(b);
+    const unsigned long long u = (unsigned long long)__double_as_longlong(r.y);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return {b, __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo))};
+}
+
 // The slope limiter of src/mhd_derivatives.f90:82-97 (MC / Van Leer, zero at extrema
 // and flat spots) from the one-sided differences dfL = (f0 - fm) / dx and
 // dfR = (fp - f0) / dx; a cell's dfR is its right neighbour's dfL, bit for bit.
-template <int LIMITER>
-__device__ __forceinline__ double limiter_of(double dfL, double dfR) {
+// SHORT: |dfL|, |dfR| <= 2^200.  Van Leer's 2 dfL dfR / (dfL + dfR) then has a
+// numerator in [2^-699, 2^401], |den| in [1e-12, 2^201] and a quotient in
+// [2^-900, 2^441] and takes the short division whenever dfL dfR >= 2^-700; a
+// positive product below that sets *tiny, and the caller must discard the result
+// and evaluate again without SHORT.
+template <int LIMITER, bool SHORT = false>
+__device__ __forceinline__ double limiter_of(double dfL, double dfR, bool *tiny = nullptr) {
     double prod = dfL * dfR;
     double den = dfL + dfR;
+    if (LIMITER == LIM_VAN_LEER && SHORT) {
+        const bool zero = prod <= 0.0 || fabs(den) < 1e-12;
+        *tiny = *tiny || (!zero && prod < 0x1p-700);
+        const double q = div_nr(2.0 * prod, recip_nr(den));
+        return zero ? 0.0 : q;
+    }
     if (prod <= 0.0 || fabs(den) < 1e-12) return 0.0;
     if (LIMITER == LIM_VAN_LEER) return (2.0 * prod) / den;
     double m = fmin(fmin(2.0 * fabs(dfL), 2.0 * fabs(dfR)), 0.5 * fabs(den));
@@ -127,16 +195,26 @@ struct Flux5 {
 
 // rusanov_flux, src/mhd_flux.f90:238-286: the physical flux of the averaged state
 // minus C/2-scaled jumps, C = max of cf + |u| on the two sides.
-__device__ __forceinline__ Flux5 rusanov(const FaceState &L, const FaceState &R, double g) {
+//
+// SHORT (the short division and square root, same bits): for face states with rho, P
+// in [1e-8, 2^66], |u|, |w|, |Bn|, |Bt| <= 2^66 and g - 1 = rg.b in [2^-100, 2^100]
+// (rg its reciprocal), P_th / (g - 1) is in [2^-140, 2^233], 1 / rho_avg in
+// [2^-66, 2^27] and c^2 = (g P_th + B^2) / rho, the square root's argument, in
+// [2^-106, 2^261].
+template <bool SHORT = false>
+__device__ __forceinline__ Flux5 rusanov(const FaceState &L, const FaceState &R, double g,
+                                         const Recip &rg = Recip{}) {
     double halfBL2 = 0.5 * (L.Bn * L.Bn + L.Bt * L.Bt);
     double halfBR2 = 0.5 * (R.Bn * R.Bn + R.Bt * R.Bt);
     double pthL = fmax(L.P - halfBL2, P_FLOOR);
     double pthR = fmax(R.P - halfBR2, P_FLOOR);
     double gm1 = g - 1.0;
-    double enL = pthL / gm1 + 0.5 * L.rho * (L.u * L.u + L.w * L.w) + halfBL2;
-    double enR = pthR / gm1 + 0.5 * R.rho * (R.u * R.u + R.w * R.w) + halfBR2;
+    double enL = (SHORT ? div_nr(pthL, rg) : pthL / gm1) +
+                 0.5 * L.rho * (L.u * L.u + L.w * L.w) + halfBL2;
+    double enR = (SHORT ? div_nr(pthR, rg) : pthR / gm1) +
+                 0.5 * R.rho * (R.u * R.u + R.w * R.w) + halfBR2;
     double rho_avg = 0.5 * (L.rho + R.rho);
-    double inv = 1.0 / rho_avg;
+    double inv = qdiv<SHORT>(1.0, rho_avg);
     double mLn = L.rho * L.u, mRn = R.rho * R.u;
     double mLt = L.rho * L.w, mRt = R.rho * R.w;
     double mx = 0.5 * (mLn + mRn);
@@ -146,10 +224,10 @@ __device__ __forceinline__ Flux5 rusanov(const FaceState &L, const FaceState &R,
     double by = 0.5 * (L.Bt + R.Bt);
     double hb = 0.5 * (bx * bx + by * by);
     double Pavg = gm1 * (en - 0.5 * (mx * mx + my * my) * inv - hb) + hb;
-    double cL2 = (g * pthL + 2.0 * halfBL2) / L.rho;
-    double cR2 = (g * pthR + 2.0 * halfBR2) / R.rho;
-    double sL = sqrt(0.5 * (cL2 + fabs(cL2))) + fabs(L.u);
-    double sR = sqrt(0.5 * (cR2 + fabs(cR2))) + fabs(R.u);
+    double cL2 = qdiv<SHORT>(g * pthL + 2.0 * halfBL2, L.rho);
+    double cR2 = qdiv<SHORT>(g * pthR + 2.0 * halfBR2, R.rho);
+    double sL = qsqrt<SHORT>(0.5 * (cL2 + fabs(cL2))) + fabs(L.u);
+    double sR = qsqrt<SHORT>(0.5 * (cR2 + fabs(cR2))) + fabs(R.u);
     double C = 0.5 * fmax(sL, sR);
     Flux5 f;
     f.mass = mx - C * (R.rho - L.rho);
@@ -162,19 +240,25 @@ __device__ __forceinline__ Flux5 rusanov(const FaceState &L, const FaceState &R,
 
 // hlle_flux, src/mhd_flux.f90:343-403: Davis speeds with c_f capped at 100, HLL
 // average, exact upwind where the fan does not straddle the face (S_R <= 0 wins).
-__device__ __forceinline__ Flux5 hlle(const FaceState &L, const FaceState &R, double g) {
+// SHORT as for rusanov; 1 / max(S_R - S_L, 1e-12) then has a divisor in
+// [1e-12, 2^68].
+template <bool SHORT = false>
+__device__ __forceinline__ Flux5 hlle(const FaceState &L, const FaceState &R, double g,
+                                      const Recip &rg = Recip{}) {
     const double cf_max = 1.0e2;
     double halfBL2 = 0.5 * (L.Bn * L.Bn + L.Bt * L.Bt);
     double halfBR2 = 0.5 * (R.Bn * R.Bn + R.Bt * R.Bt);
     double pthL = fmax(L.P - halfBL2, P_FLOOR);
     double pthR = fmax(R.P - halfBR2, P_FLOOR);
     double gm1 = g - 1.0;
-    double enL = pthL / gm1 + 0.5 * L.rho * (L.u * L.u + L.w * L.w) + halfBL2;
-    double enR = pthR / gm1 + 0.5 * R.rho * (R.u * R.u + R.w * R.w) + halfBR2;
-    double xL = (g * pthL + 2.0 * halfBL2) / L.rho;
-    double xR = (g * pthR + 2.0 * halfBR2) / R.rho;
-    double cfL = fmin(sqrt(0.5 * (xL + fabs(xL))), cf_max);
-    double cfR = fmin(sqrt(0.5 * (xR + fabs(xR))), cf_max);
+    double enL = (SHORT ? div_nr(pthL, rg) : pthL / gm1) +
+                 0.5 * L.rho * (L.u * L.u + L.w * L.w) + halfBL2;
+    double enR = (SHORT ? div_nr(pthR, rg) : pthR / gm1) +
+                 0.5 * R.rho * (R.u * R.u + R.w * R.w) + halfBR2;
+    double xL = qdiv<SHORT>(g * pthL + 2.0 * halfBL2, L.rho);
+    double xR = qdiv<SHORT>(g * pthR + 2.0 * halfBR2, R.rho);
+    double cfL = fmin(qsqrt<SHORT>(0.5 * (xL + fabs(xL))), cf_max);
+    double cfR = fmin(qsqrt<SHORT>(0.5 * (xR + fabs(xR))), cf_max);
     double SL = fmin(L.u - cfL, R.u - cfR);
     double SR = fmax(L.u + cfL, R.u + cfR);
     double mLn = L.rho * L.u, mRn = R.rho * R.u;
@@ -191,7 +275,7 @@ __device__ __forceinline__ Flux5 hlle(const FaceState &L, const FaceState &R, do
     FR.bt = R.Bt * R.u - R.Bn * R.w;
     if (SR <= 0.0) return FR;
     if (SL >= 0.0) return FL;
-    double inv = 1.0 / fmax(SR - SL, 1.0e-12);
+    double inv = qdiv<SHORT>(1.0, fmax(SR - SL, 1.0e-12));
     double sLR = SL * SR;
     Flux5 f;
     f.mass = (SR * FL.mass - SL * FR.mass + sLR * (R.rho - L.rho)) * inv;
@@ -364,10 +448,13 @@ __device__ __forceinline__ Flux5 hlld(const FaceState &L, const FaceState &R, do
     return f;
 }
 
-template <int SOLVER>
-__device__ __forceinline__ Flux5 riemann(const FaceState &L, const FaceState &R, double g) {
-    if (SOLVER == RUSANOV) return rusanov(L, R, g);
-    if (SOLVER == HLLE) return hlle(L, R, g);
+// SHORT: Rusanov and HLLE with the short division and square root (the operand ranges
+// stated at rusanov hold); HLLD always takes the full forms.
+template <int SOLVER, bool SHORT = false>
+__device__ __forceinline__ Flux5 riemann(const FaceState &L, const FaceState &R, double g,
+                                         const Recip &rg = Recip{}) {
+    if (SOLVER == RUSANOV) return rusanov<SHORT>(L, R, g, rg);
+    if (SOLVER == HLLE) return hlle<SHORT>(L, R, g, rg);
     return hlld(L, R, g);
 }
 

@@ -68,7 +68,10 @@ struct TileShape {
     static constexpr int FY_SZ = FYW * FYH;            // transverse flux, all y faces
     static constexpr int FX4_SZ = FXW * TY;            // conserved fluxes, x faces j in [0,TY-1]
     static constexpr int FY4_SZ = TX * FYH;            // conserved fluxes, y faces i in [0,TX-1]
-    static constexpr int LDS_DOUBLES = 6 * P_SZ + 4 * FX4_SZ + FX_SZ + 4 * FY4_SZ + FY_SZ;
+    // one byte per wave (up to 16): some cell of the wave's phase-1 share is out of the
+    // range of the short division and square root
+    static constexpr int FLAG_OFF = 6 * P_SZ + 4 * FX4_SZ + FX_SZ + 4 * FY4_SZ + FY_SZ;
+    static constexpr int LDS_DOUBLES = FLAG_OFF + 2;
 };
 
 // Global index of a halo position: periodic wrap or zero-gradient clamp.  Halo
@@ -382,27 +385,63 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
     const double inv_dx = 1.0 / dx;
     const double inv_2dx = 1.0 / (2.0 * dx);
     const double hdx = 0.5 * dx;
+    // Rusanov and HLLE take the short division and square root (the same bits) in a
+    // tile whose stencil cells all have rho + P + |vx| + |vy| + |Bx| + |By| <= 2^64
+    // (rho, P >= 1e-12 by the floors), with dx and g - 1 in [2^-100, 2^100].  The face
+    // states are then within 2^66 in magnitude (limited or central slopes move a state
+    // by at most one neighbour difference), rho and P at least 1e-8, and the one-sided
+    // differences / dx within 2^165: the ranges the short forms need.
+    // (Not in the diagnostic variants, whose mask bits could not be taken back.)
+    static_assert(NT % 64 == 0 && NT / 64 <= 16, "wave flags hold 16 waves");
+    constexpr bool SHORT_FORMS = SOLVER != HLLD && !BITS;
+    const bool launch_short = SHORT_FORMS && dx >= 0x1p-100 && dx <= 0x1p100 &&
+                              gam - 1.0 >= 0x1p-100 && gam <= 0x1p100;
+    bool out_of_range = false;
 
     // ---- phase 1: halo load and stage primitives -------------------------------
     auto phase1 = [&](auto vol_pow2) {
         constexpr bool P2 = decltype(vol_pow2)::value;
+        constexpr int NC = S::LW * S::PH, IT = (NC + NT - 1) / NT;
+        static_assert(NC >= NT, "every thread has a first halo cell");
         const double ivol = 1.0 / vol;   // exact when vol is a power of two
-        for (int n = tid; n < S::LW * S::PH; n += NT) {
+        // every pass's loads first (M, Mx, My, E, the two x faces and the two y faces
+        // of the cell), so that the passes share one round of memory latency
+        double raw[IT][8];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int n = tid + it * NT;
+            if (it == 0 || n < NC) {
+                const int lx = n % S::LW, ly = n / S::LW;
+                const int gi = map_index(x0 - 2 + lx, nx, px);
+                const int gj = map_index(y0 - 2 + ly, ny, py);
+                const int gim = gi > 0 ? gi - 1 : (px ? nx - 1 : 0);
+                const int gjm = gj > 0 ? gj - 1 : (py ? ny - 1 : 0);
+                const uint32_t row = (uint32_t)nx * (uint32_t)gj;
+                const uint32_t k = (row + (uint32_t)gi) * 8u;
+                const uint32_t kim = (row + (uint32_t)gim) * 8u;
+                const uint32_t kjm = ((uint32_t)nx * (uint32_t)gjm + (uint32_t)gi) * 8u;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) raw[it][q] = ldg(g.uin[q], k);
+                raw[it][4] = ldg(g.uin[4], k);
+                raw[it][5] = ldg(g.uin[4], kim);
+                raw[it][6] = ldg(g.uin[5], k);
+                raw[it][7] = ldg(g.uin[5], kjm);
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int n = tid + it * NT;
+            if (it > 0 && n >= NC) continue;
             const int lx = n % S::LW, ly = n / S::LW;
             const int idx = ly * S::PW + lx;
-            const int gi = map_index(x0 - 2 + lx, nx, px);
-            const int gj = map_index(y0 - 2 + ly, ny, py);
-            const int gim = gi > 0 ? gi - 1 : (px ? nx - 1 : 0);
-            const int gjm = gj > 0 ? gj - 1 : (py ? ny - 1 : 0);
-            const uint32_t row = (uint32_t)nx * (uint32_t)gj;
-            const uint32_t k = (row + (uint32_t)gi) * 8u;
-            const uint32_t kim = (row + (uint32_t)gim) * 8u;
-            const uint32_t kjm = ((uint32_t)nx * (uint32_t)gjm + (uint32_t)gi) * 8u;
-            const double Bx = 0.5 * (ldg(g.uin[4], k) + ldg(g.uin[4], kim));
-            const double By = 0.5 * (ldg(g.uin[5], k) + ldg(g.uin[5], kjm));
+            const double Bx = 0.5 * (raw[it][4] + raw[it][5]);
+            const double By = 0.5 * (raw[it][6] + raw[it][7]);
             double rho, vx, vy, P;
-            cons_to_prim_v<P2>(ldg(g.uin[0], k), ldg(g.uin[1], k), ldg(g.uin[2], k),
-                               ldg(g.uin[3], k), Bx, By, gam, vol, ivol, rho, vx, vy, P);
+            cons_to_prim_v<P2>(raw[it][0], raw[it][1], raw[it][2], raw[it][3], Bx, By, gam, vol,
+                               ivol, rho, vx, vy, P);
+            // a NaN or infinity fails the test too
+            out_of_range |=
+                !(rho + P + fabs(vx) + fabs(vy) + fabs(Bx) + fabs(By) <= 0x1p64);
             sP[0 * S::P_SZ + idx] = rho;
             sP[1 * S::P_SZ + idx] = vx;
             sP[2 * S::P_SZ + idx] = vy;
@@ -425,14 +464,34 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
     };
     if (pow2_value(vol)) phase1(std::true_type{});
     else phase1(std::false_type{});
+    unsigned char *wave_flag = reinterpret_cast<unsigned char *>(lds + S::FLAG_OFF);
+    {
+        const bool wave_out = __builtin_amdgcn_ballot_w64(out_of_range) != 0;
+        if ((tid & 63) == 0) wave_flag[tid >> 6] = wave_out ? 1 : 0;
+    }
     __syncthreads();
+    bool tile_short;
+    {
+        constexpr int NW = NT / 64;
+        const unsigned long long *wf = reinterpret_cast<const unsigned long long *>(wave_flag);
+        unsigned long long f = wf[0];
+        if (NW > 8) f |= wf[1];
+        if (NW < 8) f &= (1ull << (8 * NW)) - 1;
+        tile_short = __builtin_amdgcn_readfirstlane(launch_short && f == 0 ? 1 : 0) != 0;
+    }
 
     // ---- phase 2: every x face and y face the tile needs, as runs of consecutive faces
     //      along their normal (one run per thread): the slope of a run's right cell is
     //      the next face's left-cell slope, so each slope is evaluated once per run.  x
     //      and y runs share one code path (strides and field roles by direction), so a
     //      wave never splits on the direction. ------------------------------------------
-    {
+    //      A thread writes only the LDS slots of its own run, so a run evaluated again
+    //      (after a Van Leer product too small for the short division) overwrites all
+    //      it wrote the first time.
+    auto phase2 = [&](auto short_forms) -> bool {
+        constexpr bool SH = decltype(short_forms)::value;
+        const Recip rg = SH ? recip_uniform(gam - 1.0) : Recip{};
+        bool tiny = false;
         const FaceRun run = kFaceRuns<TX, TY>.r[tid];
         const int dir = run.dir;                        // 0: x faces, 1: y faces (rotated)
         const int sA = dir ? S::PW : 1;                 // LDS stride along the face normal
@@ -470,7 +529,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 const double *b = role(b0, q);
                 if (use_lim) {
                     const double dR = (b[sA] - b[0]) * inv_dx;
-                    sprev[q] = limiter_of<LIMITER>((b[0] - b[-sA]) * inv_dx, dR);
+                    sprev[q] = limiter_of<LIMITER, SH && LIMITER != LIM_VAN_LEER>((b[0] - b[-sA]) * inv_dx, dR, &tiny);
                     dprev[q] = dR;
                 } else {
                     sprev[q] = central_slope(b[-sA], b[sA], inv_2dx);
@@ -490,7 +549,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 double sr;
                 if (use_lim) {
                     const double dR = (pp - p) * inv_dx;
-                    sr = limiter_of<LIMITER>(dprev[q], dR);
+                    sr = limiter_of<LIMITER, SH && LIMITER != LIM_VAN_LEER>(dprev[q], dR, &tiny);
                     dprev[q] = dR;
                 } else {
                     sr = central_slope(c, pp, inv_2dx);
@@ -540,7 +599,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             }
             FaceState a{L[0], L[1], L[2], L[3], L[4], L[5]};
             FaceState bR{R[0], R[1], R[2], R[3], R[4], R[5]};
-            Flux5 f = riemann<SOLVER>(a, bR, gam);
+            Flux5 f = riemann<SOLVER, SH>(a, bR, gam, rg);
             // transverse flux: By flux (x faces) / Bx flux (y faces), every face
             bt[t * bt_step] = f.bt;
             // conserved fluxes: faces of the tile's own rows (x) / columns (y)
@@ -552,6 +611,17 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 c0[3 * fsz] = f.energy;
             }
         }
+        return tiny;
+    };
+    if constexpr (SHORT_FORMS) {
+        if (tile_short) {
+            if (__builtin_amdgcn_ballot_w64(phase2(std::true_type{})))
+                phase2(std::false_type{});
+        } else {
+            phase2(std::false_type{});
+        }
+    } else {
+        phase2(std::false_type{});
     }
     __syncthreads();
 
@@ -561,9 +631,19 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         const int gi = x0 + ci, gj = y0 + cj;
         if (gi >= nx || gj >= ny) return;
         const uint32_t k = ((uint32_t)nx * (uint32_t)gj + (uint32_t)gi) * 8u;
+        // the cell's global values first: one round of memory latency
+        double Uc[6], U0[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Uc[q] = ldg(g.uin[q], k);
+        if (stage > 1) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) U0[q] = ldg(g.u0[q], k);
+        }
         const double dt = *g.dt;
         const double dtdx = dt * dx;
-        // conserved fluxes: x faces FX(i, j) at [j][i+1], y faces FY(i, j) at [j+1][i]
+        // conserved fluxes: x faces FX(i, j) at [j][i+1], y faces FY(i, j) at [j+1][i];
+        // the lower faces' slots exist for every cell, so they are read unconditionally
+        // and dropped by a select at an open lower boundary
         const int xr = cj * S::FXW + (ci + 1);
         const int yr = (cj + 1) * TX + ci;
         const bool xlo_open = (gi == 0) && !px;
@@ -571,13 +651,12 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         double U[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            double v = ldg(g.uin[q], k);
-            double fr = sFX[q * S::FX4_SZ + xr];
-            if (xlo_open) v = v - dtdx * fr;
-            else v = v - dtdx * fr + dtdx * sFX[q * S::FX4_SZ + xr - 1];
-            double ft = sFY[q * S::FY4_SZ + yr];
-            if (ylo_open) v = v - dtdx * ft;
-            else v = v - dtdx * ft + dtdx * sFY[q * S::FY4_SZ + yr - TX];
+            const double fr = sFX[q * S::FX4_SZ + xr], fl = sFX[q * S::FX4_SZ + xr - 1];
+            const double ft = sFY[q * S::FY4_SZ + yr], fb = sFY[q * S::FY4_SZ + yr - TX];
+            double v = Uc[q] - dtdx * fr;
+            v = xlo_open ? v : v + dtdx * fl;
+            v = v - dtdx * ft;
+            v = ylo_open ? v : v + dtdx * fb;
             U[q] = v;
             if (STORE != STORE_NONE) stg(store->Up[q], k, v);
         }
@@ -585,14 +664,16 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         const double Bxc = sP[4 * S::P_SZ + pc], Byc = sP[5 * S::P_SZ + pc];
         conserved_floors_fast(U[0], U[1], U[2], U[3], Bxc, Byc, vol, gam);
 
-        // corner EMF: Ez(a,b) = 1/4 (-F_x(By)(a,b) - F_x(By)(a,b+1) + F_y(Bx)(a,b) + F_y(Bx)(a+1,b))
+        // corner EMF: Ez(a,b) = 1/4 (-F_x(By)(a,b) - F_x(By)(a,b+1) + F_y(Bx)(a,b) + F_y(Bx)(a+1,b));
+        // at a closed hi boundary the outer flux is the inner one (both slots are read:
+        // for the corners used below they lie inside the flux arrays)
         auto negEz = [&](int a, int b) -> double {
-            const double *fbx = sBX;
-            const double *fby = sBY;
-            double f0 = fbx[(b + 1) * S::FXW + (a + 1)];
-            double up = (!py && y0 + b == ny - 1) ? f0 : fbx[(b + 2) * S::FXW + (a + 1)];
-            double e0 = fby[(b + 1) * S::FYW + (a + 1)];
-            double rt = (!px && x0 + a == nx - 1) ? e0 : fby[(b + 1) * S::FYW + (a + 2)];
+            const double f0 = sBX[(b + 1) * S::FXW + (a + 1)];
+            const double f1 = sBX[(b + 2) * S::FXW + (a + 1)];
+            const double e0 = sBY[(b + 1) * S::FYW + (a + 1)];
+            const double e1 = sBY[(b + 1) * S::FYW + (a + 2)];
+            const double up = (!py && y0 + b == ny - 1) ? f0 : f1;
+            const double rt = (!px && x0 + a == nx - 1) ? e0 : e1;
             double Ez = 0.25 * (-f0 - up + e0 + rt);
             return -Ez;
         };
@@ -601,8 +682,8 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         else dbx = (negEz(ci, cj) - negEz(ci, cj - 1)) * inv_dx;
         if (xlo_open) dby = -((negEz(ci + 1, cj) - negEz(ci, cj)) * inv_dx);
         else dby = -((negEz(ci, cj) - negEz(ci - 1, cj)) * inv_dx);
-        double bxn = ldg(g.uin[4], k) + dt * dbx;
-        double byn = ldg(g.uin[5], k) + dt * dby;
+        double bxn = Uc[4] + dt * dbx;
+        double byn = Uc[5] + dt * dby;
 
         // outflow boundary flux correction (src/main.f90:116-148), stage primitives
         if (xlo_open || ylo_open) {
@@ -632,9 +713,9 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
             const double c0 = (stage == 2) ? 0.75 : 1.0 / 3.0;
             const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) U[q] = c0 * ldg(g.u0[q], k) + c1 * U[q];
-            bxn = c0 * ldg(g.u0[4], k) + c1 * bxn;
-            byn = c0 * ldg(g.u0[5], k) + c1 * byn;
+            for (int q = 0; q < 4; ++q) U[q] = c0 * U0[q] + c1 * U[q];
+            bxn = c0 * U0[4] + c1 * bxn;
+            byn = c0 * U0[5] + c1 * byn;
             conserved_floors_fast(U[0], U[1], U[2], U[3], Bxc, Byc, vol, gam);
         }
 #pragma unroll
