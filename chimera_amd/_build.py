@@ -88,6 +88,31 @@ def build_h5(verbose: bool = False) -> Path | None:
     return LIB_H5
 
 
+def build_checks(verbose: bool = False) -> Path | None:
+    """Build the GPU check library of the tests (tests/host/libshort_forms_check.so:
+    the stage kernel's short division and square root against hipcc's IEEE forms) with
+    the library's own flags; None when the tests are not in the tree."""
+    src = PKG.parent / "tests" / "host" / "short_forms_check.hip"
+    if not src.exists():
+        return None
+    out = src.with_name("libshort_forms_check.so")
+    tmp = out.with_suffix(".so.tmp")
+    OBJ.mkdir(exist_ok=True)
+    obj = OBJ / (src.stem + ".o")
+    # compiled and linked in two steps like the library (a one-step -shared build
+    # embeds the device code in a form torch's bundled HIP runtime does not load)
+    for cmd in ([hipcc(), *HIPCC_FLAGS, f"-I{CSRC}", "-c", str(src), "-o", str(obj)],
+                [hipcc(), "-shared", "--offload-arch=gfx950", "-fPIC", "-o", str(tmp), str(obj)]):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr}")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}", file=sys.stderr)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_h5(verbose=True)
+    build_checks(verbose=True)

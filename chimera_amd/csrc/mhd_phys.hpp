@@ -151,21 +151,10 @@ __device__ __forceinline__ Recip recip_uniform(double b) {
 // The slope limiter of src/mhd_derivatives.f90:82-97 (MC / Van Leer, zero at extrema
 // and flat spots) from the one-sided differences dfL = (f0 - fm) / dx and
 // dfR = (fp - f0) / dx; a cell's dfR is its right neighbour's dfL, bit for bit.
-// SHORT: |dfL|, |dfR| <= 2^200.  Van Leer's 2 dfL dfR / (dfL + dfR) then has a
-// numerator in [2^-699, 2^401], |den| in [1e-12, 2^201] and a quotient in
-// [2^-900, 2^441] and takes the short division whenever dfL dfR >= 2^-700; a
-// positive product below that sets *tiny, and the caller must discard the result
-// and evaluate again without SHORT.
-template <int LIMITER, bool SHORT = false>
-__device__ __forceinline__ double limiter_of(double dfL, double dfR, bool *tiny = nullptr) {
+template <int LIMITER>
+__device__ __forceinline__ double limiter_of(double dfL, double dfR) {
     double prod = dfL * dfR;
     double den = dfL + dfR;
-    if (LIMITER == LIM_VAN_LEER && SHORT) {
-        const bool zero = prod <= 0.0 || fabs(den) < 1e-12;
-        *tiny = *tiny || (!zero && prod < 0x1p-700);
-        const double q = div_nr(2.0 * prod, recip_nr(den));
-        return zero ? 0.0 : q;
-    }
     if (prod <= 0.0 || fabs(den) < 1e-12) return 0.0;
     if (LIMITER == LIM_VAN_LEER) return (2.0 * prod) / den;
     double m = fmin(fmin(2.0 * fabs(dfL), 2.0 * fabs(dfR)), 0.5 * fabs(den));

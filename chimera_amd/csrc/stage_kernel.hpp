@@ -485,13 +485,9 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
     //      the next face's left-cell slope, so each slope is evaluated once per run.  x
     //      and y runs share one code path (strides and field roles by direction), so a
     //      wave never splits on the direction. ------------------------------------------
-    //      A thread writes only the LDS slots of its own run, so a run evaluated again
-    //      (after a Van Leer product too small for the short division) overwrites all
-    //      it wrote the first time.
-    auto phase2 = [&](auto short_forms) -> bool {
+    auto phase2 = [&](auto short_forms) {
         constexpr bool SH = decltype(short_forms)::value;
         const Recip rg = SH ? recip_uniform(gam - 1.0) : Recip{};
-        bool tiny = false;
         const FaceRun run = kFaceRuns<TX, TY>.r[tid];
         const int dir = run.dir;                        // 0: x faces, 1: y faces (rotated)
         const int sA = dir ? S::PW : 1;                 // LDS stride along the face normal
@@ -529,7 +525,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 const double *b = role(b0, q);
                 if (use_lim) {
                     const double dR = (b[sA] - b[0]) * inv_dx;
-                    sprev[q] = limiter_of<LIMITER, SH && LIMITER != LIM_VAN_LEER>((b[0] - b[-sA]) * inv_dx, dR, &tiny);
+                    sprev[q] = limiter_of<LIMITER>((b[0] - b[-sA]) * inv_dx, dR);
                     dprev[q] = dR;
                 } else {
                     sprev[q] = central_slope(b[-sA], b[sA], inv_2dx);
@@ -549,7 +545,7 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 double sr;
                 if (use_lim) {
                     const double dR = (pp - p) * inv_dx;
-                    sr = limiter_of<LIMITER, SH && LIMITER != LIM_VAN_LEER>(dprev[q], dR, &tiny);
+                    sr = limiter_of<LIMITER>(dprev[q], dR);
                     dprev[q] = dR;
                 } else {
                     sr = central_slope(c, pp, inv_2dx);
@@ -611,15 +607,10 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                 c0[3 * fsz] = f.energy;
             }
         }
-        return tiny;
     };
     if constexpr (SHORT_FORMS) {
-        if (tile_short) {
-            if (__builtin_amdgcn_ballot_w64(phase2(std::true_type{})))
-                phase2(std::false_type{});
-        } else {
-            phase2(std::false_type{});
-        }
+        if (tile_short) phase2(std::true_type{});
+        else phase2(std::false_type{});
     } else {
         phase2(std::false_type{});
     }
