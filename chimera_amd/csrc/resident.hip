@@ -46,9 +46,12 @@ int check_scheme(const chimera_scheme_t *s) {
 // ---------------------------------------------------------------------------------
 // kernels
 
+// 512-thread Rusanov / HLLE blocks: at most 128 VGPRs, so that two blocks (four waves
+// per SIMD) share a CU
 template <int SOLVER, int LIMITER, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void stage_kernel(GridView g, int stage, int use_lim,
-                                                       int mood, int tiles_x, int ntiles) {
+__global__ __launch_bounds__(TX * TY)
+__attribute__((amdgpu_waves_per_eu((TX * TY == 512 && SOLVER != HLLD) ? 4 : 1)))
+void stage_kernel(GridView g, int stage, int use_lim, int mood, int tiles_x, int ntiles) {
     __shared__ double lds[TileShape<TX, TY>::LDS_DOUBLES];
     const int b = blockIdx.x, nb = gridDim.x;  // nb is a multiple of 8
     const int tile = (b % 8) * (nb / 8) + b / 8;

@@ -255,45 +255,42 @@ __device__ __forceinline__ unsigned thermal_pair(double L[6], double R[6], const
 }
 
 // Face states of one field at one face without the diagnostic bits: the MUSCL left
-// state, the right state, MOOD against the 5-point stencil of the left cell (Q1: the
-// right state too) and the rho/P floor (src/mhd_derivatives.f90:150-229).  b points at
-// the left cell c in LDS; p = b[sA] is the next cell along the face normal, the other
-// stencil members are b[-sA], b[sT] and b[-sT].  At a non-periodic hi boundary the
-// reference takes the cell value itself for the right state; here the clamped halo
-// makes p = pp = c there, so the right cell's slope is exactly 0 (limited:
-// dfL * dfR = 0; central: (c - c) / 2dx) and p - hdx * 0 = c, the same bits without a
-// test.
+// state c + dx/2 sl and right state p - dx/2 sr (src/mhd_derivatives.f90:150-229).
+// At a non-periodic hi boundary the reference takes the cell value itself for the
+// right state; here the clamped halo makes p = pp = c there, so the right cell's slope
+// is exactly 0 (limited: dfL * dfR = 0; central: (c - c) / 2dx) and p - hdx * 0 = c,
+// the same bits without a test.
 //
-// MOOD resets a state only outside [lo, hi], the extremes of the stencil, which holds
-// c and p; a state inside [min(c, p), max(c, p)] (and for rho and P at or above the
-// 1e-9 floor of lo) is therefore never reset.  The limited states of smooth data are
-// inside; only where some lane of the wave is not are the other three stencil members
-// read and the full test made (the same result for every lane).
-__device__ __forceinline__ void face_states_nb(const double *b, int sA, int sT, double c,
-                                               double p, double sl, double sr, bool mood,
-                                               bool rho_or_p, double hdx, double &fl,
-                                               double &fr) {
-    double l = c + hdx * sl;
-    double r = p - hdx * sr;
-    if (mood) {
-        const double mn = vmin(c, p), mx = vmax(c, p);
-        bool inside = l >= mn && l <= mx && r >= mn && r <= mx;
-        if (rho_or_p) inside = inside && l >= 1e-9 && r >= 1e-9;
-        if (__builtin_amdgcn_ballot_w64(!inside)) {
-            const double m = b[-sA], nA = b[sT], nB = b[-sT];
-            double lo = vmin(vmin(vmin(vmin(c, p), m), nA), nB);
-            const double hi = vmax(vmax(vmax(vmax(c, p), m), nA), nB);
-            if (rho_or_p) lo = vmax_s(lo, 1e-9);
-            if (l < lo || l > hi) l = c;
-            if (r < lo || r > hi) r = c;
-        }
-    }
-    if (rho_or_p) {
-        l = vmax_s(l, 1.0e-8);
-        r = vmax_s(r, 1.0e-8);
-    }
+// MOOD resets a state only outside [lo, hi], the extremes of the 5-point stencil of
+// the left cell (Q1: the right state too), which holds c and p; a state inside
+// [min(c, p), max(c, p)] (and for rho and P at or above the 1e-9 floor of lo) is
+// therefore never reset.  The limited states of smooth data are inside: the return
+// value says whether they are, and only in waves where some state is not does
+// mood_full read the other three stencil members and make the full test (the same
+// result for every lane).
+__device__ __forceinline__ bool face_states_inside(double c, double p, double sl, double sr,
+                                                   bool rho_or_p, double hdx, double &fl,
+                                                   double &fr) {
+    const double l = c + hdx * sl;
+    const double r = p - hdx * sr;
+    const double mn = vmin(c, p), mx = vmax(c, p);
+    bool inside = l >= mn && l <= mx && r >= mn && r <= mx;
+    if (rho_or_p) inside = inside && l >= 1e-9 && r >= 1e-9;
     fl = l;
     fr = r;
+    return inside;
+}
+// The full MOOD test of one field's states; b points at the left cell c in LDS, p =
+// b[sA] is the next cell along the face normal, the other stencil members are b[-sA],
+// b[sT] and b[-sT].
+__device__ __forceinline__ void mood_full(const double *b, int sA, int sT, double c,
+                                          bool rho_or_p, double &fl, double &fr) {
+    const double p = b[sA], m = b[-sA], nA = b[sT], nB = b[-sT];
+    double lo = vmin(vmin(vmin(vmin(c, p), m), nA), nB);
+    const double hi = vmax(vmax(vmax(vmax(c, p), m), nA), nB);
+    if (rho_or_p) lo = vmax_s(lo, 1e-9);
+    if (fl < lo || fl > hi) fl = c;
+    if (fr < lo || fr > hi) fr = c;
 }
 
 // Global loads / stores at a 32-bit byte offset from a wave-uniform base (the
@@ -362,10 +359,13 @@ __device__ __forceinline__ void conserved_floors_fast(double &M, double &Mx, dou
 
 // MASK: also OR the MOOD / thermal bits of owned faces into g.mask (the stage-1
 // diagnostics); the bits are assembled only with MASK or STORE_ALL.
-template <int SOLVER, int LIMITER, int TX, int TY, int STORE = STORE_NONE, bool MASK = false>
-__device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool use_lim, bool mood,
-                                           int tile_x, int tile_y, double *lds,
-                                           const StageStore *store = nullptr) {
+// LM: slope limiting and MOOD known to be on (the usual scheme): no per-field branch
+// on either flag.
+template <int SOLVER, int LIMITER, int TX, int TY, int STORE, bool MASK, bool LM>
+__device__ __forceinline__ void stage_tile_impl(const GridView &g, int stage, bool use_lim_arg,
+                                                bool mood_arg, int tile_x, int tile_y,
+                                                double *lds, const StageStore *store) {
+    const bool use_lim = LM || use_lim_arg, mood = LM || mood_arg;
     using S = TileShape<TX, TY>;
     constexpr int NT = TX * TY;
     constexpr bool BITS = MASK || STORE == STORE_ALL;
@@ -561,8 +561,16 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
                     if (q == F_RHO) fbits |= fb << (2 * dir);
                     if (q == F_P) fbits |= fb << (4 + 2 * dir);
                 } else {
-                    face_states_nb(b, sA, sT, c, p, sl, sr, mood, q == F_RHO || q == F_P, hdx,
-                                   L[q], R[q]);
+                    // MOOD's full test only in waves where some state is not inside
+                    // [c, p]; then the rho / P floor of the states
+                    const bool rp = q == F_RHO || q == F_P;
+                    const bool in = face_states_inside(c, p, sl, sr, rp, hdx, L[q], R[q]);
+                    if (mood && __builtin_amdgcn_ballot_w64(!in))
+                        mood_full(b, sA, sT, c, rp, L[q], R[q]);
+                    if (rp) {
+                        L[q] = vmax_s(L[q], 1.0e-8);
+                        R[q] = vmax_s(R[q], 1.0e-8);
+                    }
                 }
             }
             // thermal-pressure check (src/mhd_derivatives.f90:262-267): both faces reset
@@ -714,6 +722,18 @@ __device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool us
         stg(g.uout[4], k, bxn);
         stg(g.uout[5], k, byn);
     }
+}
+
+template <int SOLVER, int LIMITER, int TX, int TY, int STORE = STORE_NONE, bool MASK = false>
+__device__ __forceinline__ void stage_tile(const GridView &g, int stage, bool use_lim, bool mood,
+                                           int tile_x, int tile_y, double *lds,
+                                           const StageStore *store = nullptr) {
+    if (use_lim && mood)
+        stage_tile_impl<SOLVER, LIMITER, TX, TY, STORE, MASK, true>(g, stage, true, true, tile_x,
+                                                                    tile_y, lds, store);
+    else
+        stage_tile_impl<SOLVER, LIMITER, TX, TY, STORE, MASK, false>(g, stage, use_lim, mood,
+                                                                     tile_x, tile_y, lds, store);
 }
 
 }  // namespace chimera
