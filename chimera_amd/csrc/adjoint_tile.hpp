@@ -153,7 +153,7 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
     const double c1 = (stage == 2) ? 0.25 : 2.0 / 3.0;
     size_t ck[CPT];
     bool cvalid[CPT];
-    double cup[CPT][4], cu0[CPT][4], clo[CPT][6];
+    double cup[CPT][4], cu0[CPT][4], clo[CPT][6], cl0[CPT][6];
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
         const int c = tid + u * NT;
@@ -168,6 +168,15 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
             }
 #pragma unroll
             for (int q = 0; q < 6; ++q) clo[u][q] = lout[q][ck[u]];
+            // stage 2 adds to the lam_u0 that stage 3 wrote: read now with the other
+            // loads (only this thread's owned cell k reads and writes lU0 at k)
+            if (stage == 2) {
+                const int ci = c % A::CW, cj = c / A::CW;
+                if (ci < TX && cj < TY) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) cl0[u][q] = lU0[q][ck[u]];
+                }
+            }
         }
     }
 
@@ -332,18 +341,14 @@ __device__ __forceinline__ void rev_tile_a(const GridView &g, const AdjGrid &G, 
             // stage 3 starts lam_u0 (written), stage 2 adds to it
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                if (owned) {
-                    if (stage == 3) lU0[q][k] = c0 * lV[q];
-                    else lU0[q][k] += c0 * lV[q];
-                }
+                if (owned) lU0[q][k] = stage == 3 ? c0 * lV[q] : cl0[u][q] + c0 * lV[q];
                 l3[q] = c1 * lV[q];
             }
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                if (owned) {
-                    if (stage == 3) lU0[4 + q][k] = c0 * clo[u][4 + q];
-                    else lU0[4 + q][k] += c0 * clo[u][4 + q];
-                }
+                if (owned)
+                    lU0[4 + q][k] =
+                        stage == 3 ? c0 * clo[u][4 + q] : cl0[u][4 + q] + c0 * clo[u][4 + q];
                 lb[q] = c1 * clo[u][4 + q];
             }
         } else {
@@ -449,9 +454,11 @@ struct RevShapeB {
 };
 
 // lU0_add: the step's lam_u0, added to lam_in at the end (stage 1), else nullptr.
-// EARLY_TAIL issues the tail's loads with the last round (a lone small grid, whose few
-// waves per SIMD cannot hide their latency); off, they are issued where they are used,
-// which keeps 56 VGPRs out of the field loop for grids with waves enough to hide it.
+// EARLY_TAIL issues the tail's loads with the last round and the fields' loads two
+// rounds ahead (a lone small grid, whose few waves per SIMD cannot hide their
+// latency); off, the fields' loads are issued one round ahead and the tail's where
+// they are used, which keeps 84 VGPRs out of the field loop for grids with waves
+// enough to hide it.
 template <int LIMITER, int TX, int TY, bool EARLY_TAIL = true>
 __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const *U,
                                            const FusedBufs &F, double *const *lin,
@@ -509,7 +516,8 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
         t.cym = !py && j == 0;
         t.cyp = !py && j == ny - 1;
     }
-    double pv[2][PPT], rv[2][GPT][6];
+    constexpr int NBUF = EARLY_TAIL ? 3 : 2;   // rounds of loads in flight + 1
+    double pv[NBUF][PPT], rv[NBUF][GPT][6];
     auto load = [&](int f, int buf) {
 #pragma unroll
         for (int u = 0; u < PPT; ++u)
@@ -562,19 +570,19 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
 #pragma unroll
         for (int u = 0; u < B::CPT; ++u) load_cell(u);
     };
-    load(0, 0);
+#pragma unroll
+    for (int f = 0; f < NBUF - 1; ++f) load(f, f);
 
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        const int cur = f & 1;
+        const int cur = f % NBUF;
         // the field's stage primitive over cells [-2, T+2] (clamped like the forward)
 #pragma unroll
         for (int u = 0; u < PPT; ++u)
             if (tid + u * NT < B::P_SZ) sA[tid + u * NT] = pv[cur][u];
         __syncthreads();
-        if (f < 5) {
-            load(f + 1, cur ^ 1);
-        } else if (EARLY_TAIL) {
+        if (f + NBUF - 1 < 6) load(f + NBUF - 1, (f + NBUF - 1) % NBUF);
+        if (f == 5 && EARLY_TAIL) {
             load_cells();
             load_owned();
         }
