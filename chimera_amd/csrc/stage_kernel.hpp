@@ -536,6 +536,33 @@ __device__ __forceinline__ void stage_tile_impl(const GridView &g, int stage, bo
             const double *bf = b0 + t * sA;
             double L[6], R[6], C[6];
             unsigned bits = 0, fbits = 0;
+            // Van Leer (limiting on): the six fields' quotients 2 dfL dfR / (dfL + dfR)
+            // side by side in one branch, taken in waves where some field's slope is
+            // not zero (limiter_of's value, field by field)
+            double vl[6];
+            if constexpr (LM && LIMITER == LIM_VAN_LEER && !BITS) {
+                double num[6], den[6];
+                bool nz[6], any = false;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const double *b = role(bf, q);
+                    const double dR = (b[2 * sA] - b[sA]) * inv_dx;
+                    const double dL = dprev[q];
+                    const double prod = dL * dR;
+                    den[q] = dL + dR;
+                    num[q] = 2.0 * prod;
+                    nz[q] = !(prod <= 0.0 || fabs(den[q]) < 1e-12);
+                    any = any || nz[q];
+                    dprev[q] = dR;
+                }
+                if (__builtin_amdgcn_ballot_w64(any)) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) vl[q] = nz[q] ? num[q] / den[q] : 0.0;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) vl[q] = 0.0;
+                }
+            }
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 const double *b = role(bf, q);
@@ -543,7 +570,9 @@ __device__ __forceinline__ void stage_tile_impl(const GridView &g, int stage, bo
                 C[q] = c;
                 const double sl = sprev[q];
                 double sr;
-                if (use_lim) {
+                if (LM && LIMITER == LIM_VAN_LEER && !BITS) {
+                    sr = vl[q];
+                } else if (use_lim) {
                     const double dR = (pp - p) * inv_dx;
                     sr = limiter_of<LIMITER>(dprev[q], dR);
                     dprev[q] = dR;
