@@ -295,12 +295,20 @@ __device__ __forceinline__ void mood_full(const double *b, int sA, int sT, doubl
 
 // Global loads / stores at a 32-bit byte offset from a wave-uniform base (the
 // scalar-base form of the global instructions: no 64-bit address arithmetic per
-// access).  Grids of the fused path stay below 2^29 cells per field.
+// access).  Grids of the fused path stay below 2^29 cells per field.  The base is
+// taken in the global address space: every state array is device memory, and a
+// pointer the compiler cannot place (one read from a descriptor table, as in the
+// batched kernels) would otherwise become a flat access -- a 64-bit address per lane,
+// and a load that also counts against the LDS wait counter.
+typedef const char __attribute__((address_space(1))) *gchar_cptr;
+typedef char __attribute__((address_space(1))) *gchar_ptr;
 __device__ __forceinline__ double ldg(const double *base, uint32_t off) {
-    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+    const gchar_cptr p = (gchar_cptr)(reinterpret_cast<const char *>(base)) + off;
+    return *(const double __attribute__((address_space(1))) *)p;
 }
 __device__ __forceinline__ void stg(double *base, uint32_t off, double v) {
-    *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off) = v;
+    const gchar_ptr p = (gchar_ptr)(reinterpret_cast<char *>(base)) + off;
+    *(double __attribute__((address_space(1))) *)p = v;
 }
 
 // True when v is a positive power of two (division by it is then exactly a
@@ -667,7 +675,7 @@ __device__ __forceinline__ void stage_tile_impl(const GridView &g, int stage, bo
 #pragma unroll
             for (int q = 0; q < 6; ++q) U0[q] = ldg(g.u0[q], k);
         }
-        const double dt = *g.dt;
+        const double dt = ldg(g.dt, 0u);
         const double dtdx = dt * dx;
         // conserved fluxes: x faces FX(i, j) at [j][i+1], y faces FY(i, j) at [j+1][i];
         // the lower faces' slots exist for every cell, so they are read unconditionally
