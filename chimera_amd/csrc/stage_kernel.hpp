@@ -659,23 +659,28 @@ __device__ __forceinline__ void stage_tile_impl(const GridView &g, int stage, bo
     } else {
         phase2(std::false_type{});
     }
-    __syncthreads();
-
-    // ---- phase 3: update, floors, CT, outflow correction, blend (one cell per thread)
-    {
-        const int ci = tid % TX, cj = tid / TX;
-        const int gi = x0 + ci, gj = y0 + cj;
-        if (gi >= nx || gj >= ny) return;
-        const uint32_t k = ((uint32_t)nx * (uint32_t)gj + (uint32_t)gi) * 8u;
-        // the cell's global values first: one round of memory latency
-        double Uc[6], U0[6];
+    // phase 3's global loads (the cell's stage input and u^n, which this launch does not
+    // write) are issued before the barrier, so that their latency overlaps the wait for
+    // the other waves' faces; the face loop's registers are free by now
+    const int ci = tid % TX, cj = tid / TX;
+    const int gi = x0 + ci, gj = y0 + cj;
+    const bool cell_active = gi < nx && gj < ny;
+    const uint32_t k = ((uint32_t)nx * (uint32_t)gj + (uint32_t)gi) * 8u;
+    double Uc[6], U0[6];
+    if (cell_active) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) Uc[q] = ldg(g.uin[q], k);
         if (stage > 1) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) U0[q] = ldg(g.u0[q], k);
         }
-        const double dt = ldg(g.dt, 0u);
+    }
+    const double dt = ldg(g.dt, 0u);
+    __syncthreads();
+
+    // ---- phase 3: update, floors, CT, outflow correction, blend (one cell per thread)
+    {
+        if (!cell_active) return;
         const double dtdx = dt * dx;
         // conserved fluxes: x faces FX(i, j) at [j][i+1], y faces FY(i, j) at [j+1][i];
         // the lower faces' slots exist for every cell, so they are read unconditionally
