@@ -126,6 +126,9 @@ def _declare(L: C.CDLL) -> None:
         "chimera_ctx_create": [C.POINTER(vp), i, i, d, d, pBC, pSch, vp],
         "chimera_ctx_destroy": [vp],
         "chimera_ctx_upload": [vp, C.POINTER(vp)],
+        "chimera_problem_info": [i, i, C.POINTER(d), C.POINTER(d), pBC],
+        "chimera_setup_problem": [i, i, C.POINTER(vp), vp],
+        "chimera_ctx_setup_problem": [vp, i],
         "chimera_ctx_download": [vp, C.POINTER(vp)],
         "chimera_ctx_steps": [vp, i, i, c_dp, i],
         "chimera_ctx_get_time": [vp, C.POINTER(d), C.POINTER(i)],

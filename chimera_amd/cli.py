@@ -10,8 +10,9 @@ clamp) until t = tEnd = 1; a snapshot of rho, thermal P, cell B and v every
 tOut = 0.01 and at the end; the progress line with avg(|div B|) every 1% (not for
 problem 5); one HDF5 file in the reference's layout at output_path // h5_filename.
 
-Problem 5 is the reference's GRF set-up (problems.grf_problem) generated on the GPU
-with uniforms drawn from numpy's PCG64 seeded with `seed`: the reference draws them
+Problems 1-4 are set up on the GPU by the library (problems.setup_device), with the
+bits of the reference program's initial state.  Problem 5 is the reference's GRF set-up
+(problems.grf_problem) generated on the GPU with uniforms drawn from numpy's PCG64 seeded with `seed`: the reference draws them
 with the Fortran compiler's random_number, which no other build reproduces.  Its
 realized M_s and beta are written as the reference computes them; problems 1-4 write
 0 for both (the reference never sets them).  Like the reference's `stop`, an invalid
@@ -77,15 +78,10 @@ def parse_args(argv: list[str]) -> dict:
 
 def setup(problem_type: int, n: int, seed: int, target_M_s: float = -1.0,
           target_beta: float = -1.0, device="cuda"):
-    """(Problem, M_s, beta) for the reference's problem types (src/mhd_init.f90:259-291)."""
-    if problem_type == 1:
-        return problems.orszag_tang(n, scheme=SCHEME), 0.0, 0.0
-    if problem_type == 2:
-        return problems.kelvin_helmholtz(n, scheme=SCHEME), 0.0, 0.0
-    if problem_type == 3:
-        return problems.field_loop_unit(n, scheme=SCHEME), 0.0, 0.0
-    if problem_type == 4:
-        return problems.rotor(n, scheme=SCHEME), 0.0, 0.0
+    """(Problem, M_s, beta) for the reference's problem types (src/mhd_init.f90:259-291);
+    every state is generated on the GPU."""
+    if problem_type in problems.SETUP_NAMES:
+        return problems.setup_device(problem_type, n, scheme=SCHEME), 0.0, 0.0
     prob, info = problems.grf_problem(n, seed, target_M_s, target_beta, scheme=SCHEME,
                                       device=device)
     return prob, info["M_s_actual"], info["beta_actual"]

@@ -499,6 +499,25 @@ extern "C" int chimera_ctx_upload(chimera_ctx_t *c, const double *const *state) 
     return CHIMERA_OK;
 }
 
+// The initial state of problem type 1-4 written on the device (setup.hip), in place of an
+// upload; the grid must be the problem's: N x N cells, dx = 1/N, its gamma and its
+// periodicity (chimera_problem_info).
+extern "C" int chimera_ctx_setup_problem(chimera_ctx_t *c, int problem_type) {
+    CH_REQUIRE(c, "NULL argument");
+    double dx = 0.0, gamma = 0.0;
+    chimera_bc_t bc{};
+    CH_TRY(problem_info(problem_type, c->nx, &dx, &gamma, &bc));
+    int px = 0, py = 0;
+    CH_TRY(check_bc(&bc, &px, &py));
+    CH_REQUIRE(c->ny == c->nx && c->dx == dx && c->gamma == gamma && c->px == px && c->py == py,
+               "problem_type %d needs an N x N grid with dx = 1/N, gamma = %.17g and %s "
+               "boundaries (grid %d x %d, dx = %.17g, gamma = %.17g)",
+               problem_type, gamma, px ? "periodic" : "outflow", c->nx, c->ny, c->dx, c->gamma);
+    double *u[6];
+    for (int q = 0; q < 6; ++q) u[q] = c->S[c->cur][q];
+    return launch_setup_problem(problem_type, c->nx, u, c->stream);
+}
+
 extern "C" int chimera_ctx_download(chimera_ctx_t *c, double *const *state) {
     CH_REQUIRE(c && state, "NULL argument");
     size_t bytes = (size_t)c->nx * c->ny * sizeof(double);

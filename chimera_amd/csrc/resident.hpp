@@ -63,4 +63,10 @@ int launch_zero_u32(uint32_t *p, size_t n, hipStream_t stream);
 int launch_divb(const double *bx, const double *by, int nx, int ny, int px, int py, double dx,
                 unsigned long long *maxbits, double *sum, hipStream_t stream);
 
+// Initial conditions of problem types 1-4 (setup.hip): dx, gamma and boundary flags of
+// problem `problem` on N x N cells, and the set-up written into the six N x N device
+// fields u[0..5] (returns once the fields are written).
+int problem_info(int problem, int n, double *dx, double *gamma, chimera_bc_t *bc);
+int launch_setup_problem(int problem, int n, double *const *u, hipStream_t stream);
+
 }  // namespace chimera

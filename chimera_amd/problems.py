@@ -204,6 +204,28 @@ def field_loop_unit(n: int, scheme=(1, 2, 1, 1)) -> Problem:
     return Problem("field_loop_unit", st, dx, gamma, PERIODIC, tuple(scheme))
 
 
+# the reference's problem types 1-4 (src/mhd_init.f90:259-288) and their host set-ups
+SETUP_NAMES = {1: "orszag_tang", 2: "kelvin_helmholtz", 3: "field_loop_unit", 4: "rotor"}
+HOST_SETUPS = {1: orszag_tang, 2: kelvin_helmholtz, 3: field_loop_unit, 4: rotor}
+
+
+def setup_device(problem_type: int, n: int, scheme=(1, 2, 1, 1)) -> Problem:
+    """Problem type 1-4 on N x N cells with the state generated on the current GPU by
+    the library (chimera_setup_problem, on torch's current stream): the same bits as
+    the host set-up of the same problem (``HOST_SETUPS``).  The returned Problem's
+    ``state`` is a float64 tensor (6, n, n) on that GPU."""
+    import torch
+    from . import _lib
+    from .solver import problem_info
+    dx, gamma, bc = problem_info(problem_type, n)
+    st = torch.empty((6, n, n), dtype=torch.float64, device="cuda")
+    ptrs = _lib.state_ptrs(st, (6, n, n), "setup_device")
+    _lib.check(_lib.lib().chimera_setup_problem(int(problem_type), int(n), ptrs,
+                                                torch.cuda.current_stream().cuda_stream),
+               "chimera_setup_problem")
+    return Problem(SETUP_NAMES[problem_type], st, dx, gamma, tuple(bc), tuple(scheme))
+
+
 def realized_parameters(prob: Problem) -> tuple[float, float]:
     """(M_s, beta) of an initial state as the reference reports them for its random
     problem (src/mhd_init.f90:833-841): rms |v| over the sound speed of the mean state,

@@ -20,6 +20,16 @@ TILE_SHAPES = ((32, 8), (16, 16), (64, 4), (32, 32), (32, 16))
 NUM_TILE_SHAPES = len(TILE_SHAPES)
 
 
+def problem_info(problem_type: int, n: int) -> tuple[float, float, tuple]:
+    """(dx, gamma, boundary flags) of the reference's problem type 1-4 on N x N cells
+    (initialize_problem, src/mhd_init.f90:259-288)."""
+    L = _lib.lib()
+    dx, gamma, bc = C.c_double(), C.c_double(), BC()
+    check(L.chimera_problem_info(int(problem_type), int(n), C.byref(dx), C.byref(gamma),
+                                 C.byref(bc)), "chimera_problem_info")
+    return dx.value, gamma.value, (bc.xlo, bc.xhi, bc.ylo, bc.yhi)
+
+
 class Grid:
     """One grid resident on the current GPU (a chimera_ctx_t)."""
 
@@ -38,6 +48,16 @@ class Grid:
     def from_problem(cls, prob, stream=None) -> "Grid":
         g = cls(prob.nx, prob.ny, prob.dx, prob.gamma, prob.bc, prob.scheme, stream)
         g.upload(prob.state)
+        return g
+
+    @classmethod
+    def from_setup(cls, problem_type: int, n: int, scheme, stream=None) -> "Grid":
+        """An N x N grid holding the initial state of the reference's problem type 1-4
+        (Orszag-Tang, Kelvin-Helmholtz, field loop, rotor; src/mhd_init.f90:259-450),
+        generated on the device with the reference program's bits."""
+        dx, gamma, bc = problem_info(problem_type, n)
+        g = cls(n, n, dx, gamma, bc, scheme, stream)
+        check(g._L.chimera_ctx_setup_problem(g._h, int(problem_type)), "chimera_ctx_setup_problem")
         return g
 
     # ---- state transfer -------------------------------------------------------

@@ -177,6 +177,22 @@ int chimera_ctx_destroy(chimera_ctx_t *ctx);
 /* state = {Mass, Momx, Momy, Energy, b_x, b_y}; host or device pointers. */
 int chimera_ctx_upload(chimera_ctx_t *ctx, const double *const *state);
 int chimera_ctx_download(chimera_ctx_t *ctx, double *const *state);
+
+/* Initial conditions of the reference's problem types 1-4 generated on the device:
+ * initialize_problem (src/mhd_init.f90:259-288) with setup_OT_vortex (:296-311),
+ * setup_KH_instab (:319-352), setup_FL_advec (:360-391) or setup_rotor (:399-450) on
+ * the grid of src/main.f90:33-39, then get_conserved (src/main.f90:49); the state has
+ * the bits of the reference program's.  Problem type 5 (random fields) is rejected
+ * (CHIMERA_EINVAL): the GRF generator sets it up.
+ * chimera_problem_info: the problem's dx (1/N), gamma and boundary flags (any output
+ * may be NULL); host only.
+ * chimera_setup_problem: into six N x N device fields state[0..5] = {Mass, Momx, Momy,
+ * Energy, b_x, b_y}, on stream (NULL: the null stream); returns once they are written.
+ * chimera_ctx_setup_problem: into the context's state, in place of an upload; the
+ * context must have the problem's grid (N x N, dx, gamma, periodicity). */
+int chimera_problem_info(int problem_type, int n, double *dx, double *gamma, chimera_bc_t *bc);
+int chimera_setup_problem(int problem_type, int n, double *const *state, void *stream);
+int chimera_ctx_setup_problem(chimera_ctx_t *ctx, int problem_type);
 /* Run nsteps SSP-RK3 steps.  dt_in: nsteps values (host memory) for
  * CHIMERA_DT_PRESCRIBED, else NULL.  use_graph: replay a captured hipGraph per step. */
 int chimera_ctx_steps(chimera_ctx_t *ctx, int nsteps, int dt_mode, const double *dt_in,
