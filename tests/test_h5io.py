@@ -107,6 +107,21 @@ def test_athena_fixture_shape():
     assert 0.0 < g["rho"].min() < g["rho"].max() < 1.0 and g["P"].min() > 0.0
 
 
+@pytest.mark.skipif(not Path("/opt/conda/bin/h5dump").exists(), reason="h5dump not available")
+def test_writer_layout_equals_the_reference_fixture(tmp_path):
+    """A one-snapshot 500 x 500 file from this writer has the structure of the reference's
+    own snapshot-format fixture (tests/validation/ot_reference.h5, which its tests read
+    like a CHIMERA output): the same groups and datasets, FP64 little-endian, the same
+    dataspaces, contiguous storage without filters, the same fill and allocation
+    properties (`h5dump -H -p`, recorded by tests/golden/make_ot_athena.py)."""
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    from make_ot_athena import LAYOUT, h5_layout
+    path = tmp_path / "one.h5"
+    fields = {k: np.full((1, 500, 500), 1.0 + i) for i, k in enumerate(h5io.FIELDS)}
+    h5io.write_snapshots(path, fields, np.array([0.5]), 5.0 / 3.0, 0.0, 0.0)
+    assert h5_layout(path) == LAYOUT.read_text()
+
+
 # ---- the reference's main program writing through libchimera_h5 ----------------------
 
 @pytest.fixture(scope="module")
