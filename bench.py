@@ -177,10 +177,16 @@ def _leg_fl(spec: dict) -> dict:
 LEGS = {"rotor": _leg_rotor, "ot": _leg_ot, "grf": _leg_grf, "fl": _leg_fl}
 
 
+# The reference's loops open an OpenMP parallel region per routine, several per stage:
+# idle threads must spin between them (a sleeping thread's wake-up at every region
+# would be timed as the reference's cost).
+OMP_WAIT_POLICY = "ACTIVE"
+
+
 def run_leg(name: str, spec: dict, threads: int) -> dict | None:
     """One CPU leg in a child process with OMP_NUM_THREADS = threads (the OpenMP runtime
     reads it once per process)."""
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_WAIT_POLICY="PASSIVE")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_WAIT_POLICY=OMP_WAIT_POLICY)
     cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-leg",
            json.dumps({"name": name, "spec": spec})]
     print(f"[bench] cpu leg {name} threads={threads} {spec}", file=sys.stderr, flush=True)
@@ -235,7 +241,8 @@ def cpu_baseline_of(legs: dict, key: str, unit_value, sample: str, kind="referen
            "sample": sample, "seconds": a.get("seconds"),
            "one_thread": ({"value": unit_value(b), "cores": 1, "seconds": b.get("seconds")}
                           if b else None),
-           "host_nproc": host.get("nproc"), "cpu_model": host.get("cpu_model")}
+           "host_nproc": host.get("nproc"), "cpu_model": host.get("cpu_model"),
+           "omp_wait_policy": OMP_WAIT_POLICY.lower()}
     if extra:
         out.update(extra(a, b))
     return out
