@@ -74,7 +74,8 @@ class AdjointStepper:
         the states are recomputed with the fused forward and the VJPs run back to back
         on the device (one call per checkpoint segment)."""
         dts = np.ascontiguousarray(dts, dtype=np.float64)
-        assert lam.is_contiguous()
+        if not lam.is_contiguous():
+            raise ValueError("segment_vjp: lam must be contiguous (it is updated in place)")
         torch.cuda.synchronize()
         check(self._L.chimera_adj_segment_vjp(self._h, _ptrs(x0, self.shape, "x0"),
                                               dts.ctypes.data_as(C.POINTER(C.c_double)),
@@ -222,7 +223,8 @@ def _ptrs_many(ts, shapes=None, what: str = "state") -> "C.Array":
 
 def _dt_table(dts, members: int) -> np.ndarray:
     d = np.ascontiguousarray(dts, dtype=np.float64)
-    assert d.ndim == 2 and d.shape[0] == members, d.shape
+    if d.ndim != 2 or d.shape[0] != members:
+        raise ValueError(f"dt table of shape {d.shape}: need (members = {members}, steps)")
     return d
 
 
@@ -355,7 +357,9 @@ def evolve_batch(U0s, dts, badj: BatchAdjoint, checkpoint_every: int = 50):
     through the batched forward and adjoint; differentiable with respect to every U0
     (float64 CUDA tensors (6, ny, nx) of the sizes badj was made for)."""
     d = np.ascontiguousarray(dts, dtype=np.float64)
-    assert d.shape[0] == len(U0s) == badj.members
+    if d.ndim != 2 or not d.shape[0] == len(U0s) == badj.members:
+        raise ValueError(f"evolve_batch: {len(U0s)} states and a dt table of shape {d.shape} "
+                         f"for {badj.members} members")
     return _EvolveBatch.apply(badj, d, max(1, int(checkpoint_every)), *U0s)
 
 

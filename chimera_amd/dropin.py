@@ -16,7 +16,9 @@ from ._lib import BC, Scheme, check, c_dp
 
 
 def _p(a: np.ndarray):
-    assert a.dtype == np.float64 and a.flags.c_contiguous
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError(f"need a C-contiguous float64 array (got {a.dtype}, "
+                         f"contiguous={a.flags.c_contiguous})")
     return a.ctypes.data_as(c_dp)
 
 
@@ -66,9 +68,13 @@ def reconstruction(f, f_dx, f_dy, dx, bc, rho_or_p, mood=True):
 def thermal_pressure_check(faces, cells):
     """faces (24, ny, nx) in the reference order P,Bx,By,rho,vx,vy x (XL,XR,YL,YR),
     updated in place; cells (6, ny, nx) = rho, vx, vy, P, Bx, By."""
-    assert faces.dtype == np.float64 and faces.flags.c_contiguous
+    if faces.dtype != np.float64 or not faces.flags.c_contiguous or faces.ndim != 3 \
+            or faces.shape[0] != 24:
+        raise ValueError("faces: need a C-contiguous float64 array (24, ny, nx)")
     cells = _f(cells)
     ny, nx = faces.shape[1:]
+    if cells.shape != (6, ny, nx):
+        raise ValueError(f"cells of shape {cells.shape}: need (6, {ny}, {nx})")
     check(_lib.lib().chimera_thermal_pressure_check(nx, ny, *[_p(faces[k]) for k in range(24)],
                                                    *[_p(cells[k]) for k in range(6)]),
           "thermal_pressure_check")

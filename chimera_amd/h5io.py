@@ -51,11 +51,15 @@ def _dp(a: np.ndarray):
 def write_snapshots(path, fields: dict, times, gamma: float, M_s: float, beta: float) -> None:
     """fields: the six primitives (P = thermal pressure), each (nsnap, ny, nx) float64."""
     arrs = [np.ascontiguousarray(fields[k], dtype=np.float64) for k in FIELDS]
+    if arrs[0].ndim != 3:
+        raise ValueError(f"fields: need (nsnap, ny, nx) arrays, got shape {arrs[0].shape}")
     nsnap, ny, nx = arrs[0].shape
-    for a in arrs:
-        assert a.shape == (nsnap, ny, nx)
+    for k, a in zip(FIELDS, arrs):
+        if a.shape != (nsnap, ny, nx):
+            raise ValueError(f"field {k} of shape {a.shape}: need {(nsnap, ny, nx)}")
     t = np.ascontiguousarray(times, dtype=np.float64)
-    assert t.shape == (nsnap,)
+    if t.shape != (nsnap,):
+        raise ValueError(f"times of shape {t.shape}: need ({nsnap},)")
     _check(lib().chimera_h5_write_snapshots(str(path).encode(), nsnap, nx, ny, *map(_dp, arrs), _dp(t),
                                             float(gamma), float(M_s), float(beta)),
            "chimera_h5_write_snapshots")

@@ -136,6 +136,34 @@ def test_state_pointers_reject_bad_buffers():
         state_ptrs([[0.0]], (6, 8, 12))
 
 
+def test_python_entry_points_reject_bad_arrays(tmp_path):
+    """The drop-in, snapshot and adjoint helpers check the arrays the library would read
+    or write in full before calling it (and without assert, which -O strips)."""
+    from chimera_amd import adjoint, dropin, h5io
+    with pytest.raises(ValueError, match="contiguous"):
+        dropin._p(np.zeros((8, 12)).T)
+    with pytest.raises(ValueError, match="float64"):
+        dropin._p(np.zeros((8, 12), dtype=np.float32))
+    faces = np.zeros((24, 8, 12))
+    with pytest.raises(ValueError, match=r"cells of shape \(6, 7, 12\)"):
+        dropin.thermal_pressure_check(faces, np.zeros((6, 7, 12)))
+    with pytest.raises(ValueError, match="faces"):
+        dropin.thermal_pressure_check(np.zeros((23, 8, 12)), np.zeros((6, 8, 12)))
+    fields = {k: np.zeros((2, 8, 12)) for k in h5io.FIELDS}
+    fields["Vy"] = np.zeros((2, 8, 11))
+    with pytest.raises(ValueError, match="field Vy"):
+        h5io.write_snapshots(tmp_path / "x.h5", fields, np.zeros(2), 1.4, 0.0, 0.0)
+    fields["Vy"] = np.zeros((2, 8, 12))
+    with pytest.raises(ValueError, match="times"):
+        h5io.write_snapshots(tmp_path / "x.h5", fields, np.zeros(3), 1.4, 0.0, 0.0)
+    assert not (tmp_path / "x.h5").exists()
+    assert adjoint._dt_table(np.ones((3, 5)), 3).shape == (3, 5)
+    with pytest.raises(ValueError, match="members = 4"):
+        adjoint._dt_table(np.ones((3, 5)), 4)
+    with pytest.raises(ValueError, match="members = 3"):
+        adjoint._dt_table(np.ones(3), 3)
+
+
 def test_prescribed_steps_need_enough_dts(monkeypatch):
     """Grid.steps rejects a short dt sequence before calling the library."""
     from chimera_amd import solver
