@@ -26,12 +26,32 @@ def _f(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _shaped(a, shape, what: str) -> np.ndarray:
+    """a as a contiguous float64 array, which must have the shape the library reads."""
+    a = _f(a)
+    if a.shape != tuple(shape):
+        raise ValueError(f"{what} of shape {a.shape}: need {tuple(shape)}")
+    return a
+
+
+def _grid(a, what: str) -> tuple[int, int]:
+    if a.ndim != 2:
+        raise ValueError(f"{what}: need a 2-D (ny, nx) array, got shape {a.shape}")
+    return a.shape
+
+
+def _fields(a, count: int, what: str) -> tuple[int, ...]:
+    if a.ndim < 2 or a.shape[0] != count:
+        raise ValueError(f"{what}: need {count} fields along the first axis, got shape {a.shape}")
+    return a.shape[1:]
+
+
 def fill_ghost_cells(f, bc, f_ambient=None) -> np.ndarray:
     """src/mhd_bc.f90:36-141 -> (ny+2, nx+2) padded field."""
     f = _f(f)
-    ny, nx = f.shape
+    ny, nx = _grid(f, "f")
     pad = np.empty((ny + 2, nx + 2))
-    amb = _f(f_ambient) if f_ambient is not None else None
+    amb = _shaped(f_ambient, (ny, nx), "f_ambient") if f_ambient is not None else None
     check(_lib.lib().chimera_fill_ghost_cells(_p(f), _p(pad), _p(amb) if amb is not None else None,
                                              nx, ny, C.byref(BC.of(bc))), "fill_ghost_cells")
     return pad
@@ -39,7 +59,7 @@ def fill_ghost_cells(f, bc, f_ambient=None) -> np.ndarray:
 
 def compute_gradients(f_pad, dx):
     f_pad = _f(f_pad)
-    ny, nx = f_pad.shape[0] - 2, f_pad.shape[1] - 2
+    ny, nx = (m - 2 for m in _grid(f_pad, "f_pad"))
     gx, gy = np.empty((ny, nx)), np.empty((ny, nx))
     check(_lib.lib().chimera_compute_gradients(_p(f_pad), float(dx), nx, ny, _p(gx), _p(gy)),
           "compute_gradients")
@@ -48,7 +68,7 @@ def compute_gradients(f_pad, dx):
 
 def apply_slope_limiter(f_pad, dx, limiter):
     f_pad = _f(f_pad)
-    ny, nx = f_pad.shape[0] - 2, f_pad.shape[1] - 2
+    ny, nx = (m - 2 for m in _grid(f_pad, "f_pad"))
     gx, gy = np.empty((ny, nx)), np.empty((ny, nx))
     check(_lib.lib().chimera_apply_slope_limiter(_p(f_pad), float(dx), nx, ny, _p(gx), _p(gy),
                                                 int(limiter)), "apply_slope_limiter")
@@ -56,8 +76,9 @@ def apply_slope_limiter(f_pad, dx, limiter):
 
 
 def reconstruction(f, f_dx, f_dy, dx, bc, rho_or_p, mood=True):
-    f, f_dx, f_dy = _f(f), _f(f_dx), _f(f_dy)
-    ny, nx = f.shape
+    f = _f(f)
+    ny, nx = _grid(f, "f")
+    f_dx, f_dy = _shaped(f_dx, (ny, nx), "f_dx"), _shaped(f_dy, (ny, nx), "f_dy")
     out = [np.empty((ny, nx)) for _ in range(4)]
     check(_lib.lib().chimera_reconstruction(_p(f), _p(f_dx), _p(f_dy), float(dx), nx, ny,
                                            *[_p(o) for o in out], int(bool(rho_or_p)),
@@ -71,10 +92,8 @@ def thermal_pressure_check(faces, cells):
     if faces.dtype != np.float64 or not faces.flags.c_contiguous or faces.ndim != 3 \
             or faces.shape[0] != 24:
         raise ValueError("faces: need a C-contiguous float64 array (24, ny, nx)")
-    cells = _f(cells)
     ny, nx = faces.shape[1:]
-    if cells.shape != (6, ny, nx):
-        raise ValueError(f"cells of shape {cells.shape}: need (6, {ny}, {nx})")
+    cells = _shaped(cells, (6, ny, nx), "cells")
     check(_lib.lib().chimera_thermal_pressure_check(nx, ny, *[_p(faces[k]) for k in range(24)],
                                                    *[_p(cells[k]) for k in range(6)]),
           "thermal_pressure_check")
@@ -84,7 +103,7 @@ def compute_fluxes(fin, gamma, solver) -> np.ndarray:
     """fin (12, ...) = rho_L,rho_R,vx_L,vx_R,vy_L,vy_R,P_L,P_R,Bx_L,Bx_R,By_L,By_R ->
     (5, ...) = Mass, Momx, Momy, Energy, By."""
     fin = _f(fin)
-    shape = fin.shape[1:]
+    shape = _fields(fin, 12, "fin")
     n = int(np.prod(shape))
     a = fin.reshape(12, n)
     out = np.empty((5, n))
@@ -96,17 +115,18 @@ def compute_fluxes(fin, gamma, solver) -> np.ndarray:
 
 def update_conserved(F, fX, fY, dx, dt, bc):
     F = _f(F).copy()
-    fX, fY = _f(fX), _f(fY)
-    ny, nx = F.shape
+    ny, nx = _grid(F, "F")
+    fX, fY = _shaped(fX, (ny, nx), "flux_F_X"), _shaped(fY, (ny, nx), "flux_F_Y")
     check(_lib.lib().chimera_update_conserved(_p(F), _p(fX), _p(fY), float(dx), float(dt), nx, ny,
                                              C.byref(BC.of(bc))), "update_conserved")
     return F
 
 
 def constrained_transport(bx, by, fByX, fBxY, dx, dt, bc):
-    bx, by = _f(bx).copy(), _f(by).copy()
-    fByX, fBxY = _f(fByX), _f(fBxY)
-    ny, nx = bx.shape
+    bx = _f(bx).copy()
+    ny, nx = _grid(bx, "bx")
+    by = _shaped(by, (ny, nx), "by").copy()
+    fByX, fBxY = _shaped(fByX, (ny, nx), "flux_By_X"), _shaped(fBxY, (ny, nx), "flux_Bx_Y")
     check(_lib.lib().chimera_constrained_transport(_p(bx), _p(by), _p(fByX), _p(fBxY), float(dx),
                                                   float(dt), nx, ny, C.byref(BC.of(bc))),
           "constrained_transport")
@@ -115,7 +135,7 @@ def constrained_transport(bx, by, fByX, fBxY, dx, dt, bc):
 
 def compute_curl_2d(Az, dx, bc):
     Az = _f(Az)
-    ny, nx = Az.shape
+    ny, nx = _grid(Az, "Az")
     bx, by = np.empty((ny, nx)), np.empty((ny, nx))
     check(_lib.lib().chimera_compute_curl_2d(_p(Az), float(dx), nx, ny, _p(bx), _p(by),
                                             C.byref(BC.of(bc))), "compute_curl_2d")
@@ -123,8 +143,9 @@ def compute_curl_2d(Az, dx, bc):
 
 
 def compute_divB(bx, by, dx, bc):
-    bx, by = _f(bx), _f(by)
-    ny, nx = bx.shape
+    bx = _f(bx)
+    ny, nx = _grid(bx, "bx")
+    by = _shaped(by, (ny, nx), "by")
     d = np.empty((ny, nx))
     check(_lib.lib().chimera_compute_divB(_p(bx), _p(by), float(dx), nx, ny, _p(d),
                                          C.byref(BC.of(bc))), "compute_divB")
@@ -132,8 +153,9 @@ def compute_divB(bx, by, dx, bc):
 
 
 def average_face_to_cell_B(b_x, b_y, bc):
-    b_x, b_y = _f(b_x), _f(b_y)
-    ny, nx = b_x.shape
+    b_x = _f(b_x)
+    ny, nx = _grid(b_x, "b_x")
+    b_y = _shaped(b_y, (ny, nx), "b_y")
     Bx, By = np.empty((ny, nx)), np.empty((ny, nx))
     check(_lib.lib().chimera_average_face_to_cell_B(_p(b_x), _p(b_y), nx, ny, _p(Bx), _p(By),
                                                    C.byref(BC.of(bc))), "average_face_to_cell_B")
@@ -142,7 +164,7 @@ def average_face_to_cell_B(b_x, b_y, bc):
 
 def get_conserved(prim, gamma, vol) -> np.ndarray:
     prim = _f(prim)
-    shape = prim.shape[1:]
+    shape = _fields(prim, 6, "prim")
     n = int(np.prod(shape))
     out = np.empty((4,) + shape)
     check(_lib.lib().chimera_get_conserved(*[_p(prim[k]) for k in range(6)], float(gamma),
@@ -152,8 +174,9 @@ def get_conserved(prim, gamma, vol) -> np.ndarray:
 
 
 def get_primitive(cons, Bx, By, gamma, vol) -> np.ndarray:
-    cons, Bx, By = _f(cons), _f(Bx), _f(By)
-    shape = cons.shape[1:]
+    cons = _f(cons)
+    shape = _fields(cons, 4, "cons")
+    Bx, By = _shaped(Bx, shape, "Bx"), _shaped(By, shape, "By")
     n = int(np.prod(shape))
     out = np.empty((4,) + shape)
     check(_lib.lib().chimera_get_primitive(*[_p(cons[k]) for k in range(4)], _p(Bx), _p(By),
@@ -164,8 +187,9 @@ def get_primitive(cons, Bx, By, gamma, vol) -> np.ndarray:
 
 def apply_conserved_floors(cons, Bx, By, vol, gamma) -> np.ndarray:
     cons = _f(cons).copy()
-    Bx, By = _f(Bx), _f(By)
-    n = int(np.prod(cons.shape[1:]))
+    shape = _fields(cons, 4, "cons")
+    Bx, By = _shaped(Bx, shape, "Bx"), _shaped(By, shape, "By")
+    n = int(np.prod(shape))
     check(_lib.lib().chimera_apply_conserved_floors(*[_p(cons[k]) for k in range(4)], _p(Bx),
                                                    _p(By), float(vol), float(gamma), n, 1),
           "apply_conserved_floors")
@@ -176,7 +200,9 @@ def stage_fluxes(prim, dx, gamma, bc, scheme, want_mask=False):
     """compute_stage_fluxes (src/main.f90:346-403) -> (10, ny, nx) fluxes
     (Mass_X, Mass_Y, Momx_X, Momx_Y, Momy_X, Momy_Y, Energy_X, Energy_Y, By_X, Bx_Y)."""
     prim = _f(prim)
-    _, ny, nx = prim.shape
+    if prim.ndim != 3:
+        raise ValueError(f"prim: need (6, ny, nx), got shape {prim.shape}")
+    ny, nx = _fields(prim, 6, "prim")
     flux = np.empty((10, ny, nx))
     mask = np.zeros((ny, nx), dtype=np.uint32) if want_mask else None
     check(_lib.lib().chimera_stage_fluxes(_p(prim), _p(flux),

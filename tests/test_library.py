@@ -149,6 +149,24 @@ def test_python_entry_points_reject_bad_arrays(tmp_path):
         dropin.thermal_pressure_check(faces, np.zeros((6, 7, 12)))
     with pytest.raises(ValueError, match="faces"):
         dropin.thermal_pressure_check(np.zeros((23, 8, 12)), np.zeros((6, 8, 12)))
+    f, short = np.zeros((8, 12)), np.zeros((8, 11))
+    bc = (1, 1, 1, 1)
+    for call, what in ((lambda: dropin.fill_ghost_cells(f, bc, short), "f_ambient"),
+                       (lambda: dropin.reconstruction(f, f, short, 0.1, bc, 0), "f_dy"),
+                       (lambda: dropin.update_conserved(f, short, f, 0.1, 0.1, bc), "flux_F_X"),
+                       (lambda: dropin.constrained_transport(f, short, f, f, 0.1, 0.1, bc), "by"),
+                       (lambda: dropin.compute_divB(f, short, 0.1, bc), "by"),
+                       (lambda: dropin.average_face_to_cell_B(f, short, bc), "b_y"),
+                       (lambda: dropin.compute_gradients(np.zeros(10), 0.1), "f_pad"),
+                       (lambda: dropin.get_conserved(np.zeros((5, 8, 12)), 1.4, 0.1), "prim"),
+                       (lambda: dropin.get_primitive(np.zeros((4, 8, 12)), f, short, 1.4, 0.1), "By"),
+                       (lambda: dropin.apply_conserved_floors(np.zeros((3, 8, 12)), f, f, 0.1, 1.4),
+                        "cons"),
+                       (lambda: dropin.compute_fluxes(np.zeros((11, 20)), 1.4, 1), "fin"),
+                       (lambda: dropin.stage_fluxes(np.zeros((5, 8, 12)), 0.1, 1.4, bc, (1, 2)),
+                        "prim")):
+        with pytest.raises(ValueError, match=what):
+            call()
     fields = {k: np.zeros((2, 8, 12)) for k in h5io.FIELDS}
     fields["Vy"] = np.zeros((2, 8, 11))
     with pytest.raises(ValueError, match="field Vy"):
