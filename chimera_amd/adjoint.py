@@ -249,24 +249,31 @@ class BatchAdjoint:
                                     C.cast(C.pointer(sch), C.c_void_p), None), "chimera_badj_create")
         self._h = h
 
+    def _member_ptrs(self, ts, what: str) -> "C.Array":
+        """Field pointers of one state per member (the library reads 6 x members)."""
+        ts = list(ts)
+        if len(ts) != self.members:
+            raise ValueError(f"{what}: {len(ts)} states for {self.members} members")
+        return _ptrs_many(ts, self.shapes, what)
+
     def forward(self, states, dts, out=None):
         """Every member len(dts[m]) steps with its dts, from states to out (new tensors
         unless given; may be the inputs)."""
         d = _dt_table(dts, self.members)
         out = [torch.empty_like(x) for x in states] if out is None else out
         torch.cuda.synchronize()
-        check(self._L.chimera_badj_forward(self._h, _ptrs_many(states, self.shapes),
+        check(self._L.chimera_badj_forward(self._h, self._member_ptrs(states, "states"),
                                            d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
-                                           _ptrs_many(out, self.shapes)), "chimera_badj_forward")
+                                           self._member_ptrs(out, "out")), "chimera_badj_forward")
         return out
 
     def segment_vjp(self, states0, dts, lams) -> None:
         """lams[m] <- (dU^k / dU^0)^T lams[m] in place for every member, k = len(dts[m])."""
         d = _dt_table(dts, self.members)
         torch.cuda.synchronize()
-        check(self._L.chimera_badj_segment_vjp(self._h, _ptrs_many(states0, self.shapes),
+        check(self._L.chimera_badj_segment_vjp(self._h, self._member_ptrs(states0, "states0"),
                                                d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
-                                               _ptrs_many(lams, self.shapes)), "chimera_badj_segment_vjp")
+                                               self._member_ptrs(lams, "lams")), "chimera_badj_segment_vjp")
 
     def sweep(self, ckpts, dts, seg_steps: int, lams) -> None:
         """lams[m] <- (dU_N / dU_0)^T lams[m] in place for a run of dts.shape[1] steps
@@ -279,7 +286,7 @@ class BatchAdjoint:
         torch.cuda.synchronize()
         check(self._L.chimera_badj_sweep(self._h, _ptrs_many([t for c in ckpts for t in c], self.shapes),
                                          d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
-                                         int(seg_steps), _ptrs_many(lams, self.shapes)), "chimera_badj_sweep")
+                                         int(seg_steps), self._member_ptrs(lams, "lams")), "chimera_badj_sweep")
 
     def record(self, states0, dts, seg_steps: int, nseg: int, out=None):
         """dts.shape[1] (<= seg_steps) steps of every member from states0 into out (new
@@ -289,9 +296,9 @@ class BatchAdjoint:
         d = _dt_table(dts, self.members)
         out = [torch.empty_like(x) for x in states0] if out is None else out
         torch.cuda.synchronize()
-        check(self._L.chimera_badj_record(self._h, _ptrs_many(states0, self.shapes),
+        check(self._L.chimera_badj_record(self._h, self._member_ptrs(states0, "states0"),
                                           d.ctypes.data_as(C.POINTER(C.c_double)), d.shape[1],
-                                          int(seg_steps), int(nseg), _ptrs_many(out, self.shapes)),
+                                          int(seg_steps), int(nseg), self._member_ptrs(out, "out")),
               "chimera_badj_record")
         return out
 
@@ -310,7 +317,7 @@ class BatchAdjoint:
         best = (C.c_int * 3)()
         ms = (C.c_double * 6)()
         torch.cuda.synchronize()
-        check(self._L.chimera_badj_autotune(self._h, _ptrs_many(states, self.shapes),
+        check(self._L.chimera_badj_autotune(self._h, self._member_ptrs(states, "states"),
                                             d.ctypes.data_as(C.POINTER(C.c_double)), int(reps),
                                             best, ms), "chimera_badj_autotune")
         return tuple(best), list(ms)

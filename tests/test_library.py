@@ -175,6 +175,10 @@ def test_python_entry_points_reject_bad_arrays(tmp_path):
     with pytest.raises(ValueError, match="times"):
         h5io.write_snapshots(tmp_path / "x.h5", fields, np.zeros(3), 1.4, 0.0, 0.0)
     assert not (tmp_path / "x.h5").exists()
+    badj = adjoint.BatchAdjoint.__new__(adjoint.BatchAdjoint)   # no device context
+    badj.members, badj.shapes, badj._h = 3, [(8, 8)] * 3, None
+    with pytest.raises(ValueError, match="2 states for 3 members"):
+        badj._member_ptrs([None, None], "states")
     assert adjoint._dt_table(np.ones((3, 5)), 3).shape == (3, 5)
     with pytest.raises(ValueError, match="members = 4"):
         adjoint._dt_table(np.ones((3, 5)), 4)
