@@ -714,9 +714,9 @@ def main():
         print(json.dumps(LEGS[leg["name"]](leg["spec"])))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding of the "
-                          "stage update is a design (see the sharding notes), not a bench "
-                          "configuration", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on one GPU only: the stage update is "
+                          "not sharded across GPUs (one GPU's strip of a sharded grid is the "
+                          "shard_strip section)", "n_gpus": args.gpus}))
         sys.exit(2)
 
     tmpdir = tempfile.mkdtemp(prefix="chimera_bench_")
