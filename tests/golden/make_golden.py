@@ -8,7 +8,7 @@ oracle/build_ref.sh with amdflang, no FMA contraction), on small seeded inputs:
   - compute_stage_fluxes on a seeded primitive field (periodic and outflow);
   - compute_fluxes on seeded face states for each Riemann solver.
 They let the oracle be checked against the reference where the reference build is
-not available (e.g. on the GPU box).  Regenerate with
+not available (a machine without the reference sources or amdflang).  Regenerate with
     python tests/golden/make_golden.py
 """
 from __future__ import annotations
