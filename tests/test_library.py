@@ -57,6 +57,26 @@ def test_library_exports_every_declared_symbol(header):
         getattr(handle, name)     # resolves through dlsym
 
 
+def test_headers_compile_as_c_and_link(tmp_path):
+    """Both headers are plain C (C99, pedantic, no warnings): a C caller includes them
+    and links against the two libraries."""
+    src = tmp_path / "caller.c"
+    src.write_text("#include \"chimera_hip.h\"\n#include \"chimera_h5.h\"\n"
+                   "int main(void) {\n"
+                   "    double dx, g; chimera_bc_t bc; chimera_ctx_t *ctx = 0;\n"
+                   "    (void)ctx;\n"
+                   "    return chimera_problem_info(1, 64, &dx, &g, &bc);\n}\n")
+    exe = tmp_path / "caller"
+    lib = ROOT / "chimera_amd"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                        f"-I{ROOT / 'include'}", str(src), f"-L{lib}", "-lchimera_hip",
+                        "-lchimera_h5", f"-Wl,-rpath,{lib}", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    # chimera_problem_info is host-only: the C caller runs here without a GPU
+    assert subprocess.run([str(exe)], capture_output=True, timeout=60).returncode == 0
+
+
 def test_invalid_arguments_rejected_on_host(L):
     from chimera_amd._lib import BC, Scheme
     h = C.c_void_p()
