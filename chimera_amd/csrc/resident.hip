@@ -47,10 +47,12 @@ int check_scheme(const chimera_scheme_t *s) {
 // kernels
 
 // 512-thread Rusanov / HLLE blocks: at most 128 VGPRs, so that two blocks (four waves
-// per SIMD) share a CU
+// per SIMD) share a CU.  256-thread HLLD blocks: at most 168 (three waves per SIMD, as
+// their LDS allows); the compiler's own 170 left two (32 x 8 and 64 x 4 8-9% slower).
 template <int SOLVER, int LIMITER, int TX, int TY>
 __global__ __launch_bounds__(TX * TY)
-__attribute__((amdgpu_waves_per_eu((TX * TY == 512 && SOLVER != HLLD) ? 4 : 1)))
+__attribute__((amdgpu_waves_per_eu((TX * TY == 512 && SOLVER != HLLD) ? 4
+                                   : ((TX * TY == 256 && SOLVER == HLLD) ? 3 : 1))))
 void stage_kernel(GridView g, int stage, int use_lim, int mood, int tiles_x, int ntiles) {
     __shared__ double lds[TileShape<TX, TY>::LDS_DOUBLES];
     const int b = blockIdx.x, nb = gridDim.x;  // nb is a multiple of 8
