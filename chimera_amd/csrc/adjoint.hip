@@ -67,10 +67,13 @@ struct RevTile {
 constexpr int REV_NUM_SHAPES = 4;
 constexpr RevTile REV_SHAPES[REV_NUM_SHAPES] = {{16, 16}, {16, 8}, {8, 8}, {32, 8}};
 
+// HLLD at three waves per SIMD (at most 168 registers, as the tile's LDS allows) but for
+// the stored-intermediate cross-check (STORE_ALL, 193 registers)
 template <int SOLVER, int LIMITER, int STORE>
-__global__ __launch_bounds__(ATX * ATY) void adj_stage_kernel(GridView g, StageStore st,
-                                                                 int stage, int use_lim, int mood,
-                                                                 int tiles_x, int ntiles) {
+__global__ __launch_bounds__(ATX * ATY)
+__attribute__((amdgpu_waves_per_eu((SOLVER == HLLD && STORE != STORE_ALL) ? 3 : 1)))
+void adj_stage_kernel(GridView g, StageStore st, int stage, int use_lim, int mood, int tiles_x,
+                      int ntiles) {
     __shared__ double lds[TileShape<ATX, ATY>::LDS_DOUBLES];
     const int b = blockIdx.x, nb = gridDim.x;  // nb is a multiple of 8
     const int tile = (b % 8) * (nb / 8) + b / 8;

@@ -89,12 +89,13 @@ __device__ __forceinline__ GridView member_view(const BAdjMember &m, int s, int 
 // trajectory.  Plain forward (PURE): the running state is the member's state_out (U^0
 // read from state0), the two intermediate stage states use trajectory slots 1 and 2,
 // and stage 3 overwrites state_out in place (it reads u^n only at the cell it writes).
-// The segment recompute (!PURE) at three waves per SIMD (at most 168 registers a lane):
-// HLLD's needs 181, and the two spilled to scratch cost less than the third wave gains
-// (the HLLD gradient 3% faster).  The plain forward keeps the compiler's choice.
+// HLLD at three waves per SIMD (at most 168 registers a lane, as the 256-thread tiles'
+// LDS allows): the compiler's 181 left two, and the two registers spilled to scratch
+// cost far less than the third wave gains (forward 21% faster, the segment recompute
+// inside the gradient 3%).  Rusanov and HLLE need ~135 and run three or more anyway.
 template <int SOLVER, int LIMITER, int TX, int TY, bool PURE>
 __global__ __launch_bounds__(STAGE_THREADS)
-__attribute__((amdgpu_waves_per_eu(PURE ? 1 : 3))) void badj_forward_kernel(
+__attribute__((amdgpu_waves_per_eu(SOLVER == HLLD ? 3 : 1))) void badj_forward_kernel(
     const BAdjMember *__restrict__ mem, const BTile *__restrict__ tiles, int ntiles, int s, int st,
     int use_lim, int mood) {
     __shared__ double lds[TileShape<TX, TY>::LDS_DOUBLES];
