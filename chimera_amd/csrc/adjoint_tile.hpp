@@ -461,9 +461,9 @@ struct RevShapeB {
 // lU0_add: the step's lam_u0, added to lam_in at the end (stage 1), else nullptr.
 // EARLY_TAIL issues the tail's loads with the last round and the fields' loads two
 // rounds ahead (a lone small grid, whose few waves per SIMD cannot hide their
-// latency); off, the fields' loads are issued one round ahead and the tail's where
-// they are used, which keeps 84 VGPRs out of the field loop for grids with waves
-// enough to hide it.
+// latency); off, each field's loads are issued at the start of its own round and the
+// tail's where they are used, which keeps ~110 VGPRs out of the field loop for the
+// batched kernel, whose four waves per SIMD hide the latency instead.
 template <int LIMITER, int TX, int TY, bool EARLY_TAIL = true>
 __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const *U,
                                            const FusedBufs &F, double *const *lin,
@@ -521,7 +521,7 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
         t.cym = !py && j == 0;
         t.cyp = !py && j == ny - 1;
     }
-    constexpr int NBUF = EARLY_TAIL ? 3 : 2;   // rounds of loads in flight + 1
+    constexpr int NBUF = EARLY_TAIL ? 3 : 1;   // rounds of loads in flight + 1
     double pv[NBUF][PPT], rv[NBUF][GPT][6];
     auto load = [&](int f, int buf) {
 #pragma unroll
@@ -581,12 +581,13 @@ __device__ __forceinline__ void rev_tile_b(const AdjGrid &G, const double *const
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
         const int cur = f % NBUF;
+        if constexpr (NBUF == 1) load(f, 0);
         // the field's stage primitive over cells [-2, T+2] (clamped like the forward)
 #pragma unroll
         for (int u = 0; u < PPT; ++u)
             if (tid + u * NT < B::P_SZ) sA[tid + u * NT] = pv[cur][u];
         __syncthreads();
-        if (f + NBUF - 1 < 6) load(f + NBUF - 1, (f + NBUF - 1) % NBUF);
+        if (NBUF > 1 && f + NBUF - 1 < 6) load(f + NBUF - 1, (f + NBUF - 1) % NBUF);
         if (f == 5 && EARLY_TAIL) {
             load_cells();
             load_owned();

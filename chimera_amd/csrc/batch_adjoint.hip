@@ -168,8 +168,11 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     }
 }
 
+// Four waves per SIMD (at most 128 registers): with one field round of loads in flight
+// (rev_tile_b without EARLY_TAIL) the kernel needs 131 and spills two; the fourth wave
+// hides the loads' latency better than the prefetch did (the HLLD gradient 3% faster).
 template <int LIMITER, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void badj_rev_b_kernel(const BAdjMember *__restrict__ mem,
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4))) void badj_rev_b_kernel(const BAdjMember *__restrict__ mem,
                                                            const BTile *__restrict__ tiles,
                                                            int ntiles, int s, int st) {
     __shared__ double lds[RevShapeB<TX, TY>::LDS_DOUBLES];
