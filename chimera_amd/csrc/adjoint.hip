@@ -345,12 +345,14 @@ __global__ void ar_bavg(AdjGrid G, RevBufs R, double *const *lin) {
 
 // ---- fused reverse kernels (one tile per workgroup, XCD-aware tile map) --------------
 
+// HLLD at two waves per SIMD: its face reverse needs ~330 registers a lane and held one
+// wave; the excess spills to scratch (OT 512^2 segment VJP 3% faster, same bits).
 template <int SOLVER, int LIMITER, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void ar_tile_a(GridView g, AdjGrid G, FusedBufs F,
-                                                   const double *const *Up,
-                                                   const double *const *lout, double *const *lin,
-                                                   double *const *lU0, int stage, double dt,
-                                                   int tiles_x, int ntiles) {
+__global__ __launch_bounds__(TX *TY)
+__attribute__((amdgpu_waves_per_eu(SOLVER == HLLD ? 2 : 1)))
+void ar_tile_a(GridView g, AdjGrid G, FusedBufs F, const double *const *Up,
+               const double *const *lout, double *const *lin, double *const *lU0, int stage,
+               double dt, int tiles_x, int ntiles) {
     __shared__ double lds[RevShapeA<TX, TY>::LDS_DOUBLES];
     const int b = blockIdx.x, nb = gridDim.x;   // nb is a multiple of 8
     const int tile = (b % 8) * (nb / 8) + b / 8;
