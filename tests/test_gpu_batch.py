@@ -60,8 +60,8 @@ def test_batch_reference_dt_policy_and_autotune(gpu_lib, orc):
                                           (4, (2, 1, 1, 1)), (4, (3, 2, 1, 1)),
                                           (4, (1, 2, 0, 0))])
 def test_single_grid_tile_shapes(gpu_lib, shape, scheme):
-    """Every tile shape gives the 32 x 8 result bit for bit (72^2: partial tiles; the
-    32 x 32 and 32 x 16 shapes on a grid smaller than three tiles per side)."""
+    """Every tile shape gives the default shape's result bit for bit (72^2: partial
+    tiles; the 32 x 32 and 32 x 16 shapes on a grid smaller than three tiles per side)."""
     from chimera_amd.solver import Grid
     prob = problems.rotor(72, scheme=scheme)
     with Grid.from_problem(prob) as g:
@@ -79,12 +79,12 @@ def test_single_grid_tile_shapes(gpu_lib, shape, scheme):
                                          ("rotor", (3, 2, 1, 1))])
 def test_default_tile_on_large_grids_matches_oracle(gpu_lib, orc, make, scheme):
     """A 512^2 grid has a 32 x 16 tile per CU, so a new context runs Rusanov and HLLE
-    on 32 x 16 tiles (HLLD keeps 32 x 8); two CFL steps equal the oracle's bit for bit."""
+    on 32 x 16 tiles (HLLD on 16 x 16); two CFL steps equal the oracle's bit for bit."""
     from chimera_amd.solver import Grid
     prob = getattr(problems, make)(512, scheme=scheme)
     print(f"default tile {make} scheme={scheme} 512^2", flush=True)
     with Grid.from_problem(prob) as g:
-        assert g.tile_shape()[0] == (0 if scheme[0] == 3 else 4)
+        assert g.tile_shape()[0] == (1 if scheme[0] == 3 else 4)
         g.steps(2, dt_mode=1)
         out = g.download()
     ref = prob.state.copy()
