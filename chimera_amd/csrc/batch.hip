@@ -404,6 +404,11 @@ extern "C" int chimera_batch_create(chimera_batch_t **out, int members, const in
         return fail(CHIMERA_EHIP);
     }
     int rc = batch_tables(b);
+    // default tile (as a single grid's): 32 x 16 once the ensemble has a tile per CU and
+    // the solver leaves room for two such blocks per CU (not HLLD), HLLD 16 x 16, else
+    // 32 x 8; chimera_batch_set_tiling / chimera_batch_autotune change it
+    const bool hlld = b->sch.riemann_solver == HLLD;
+    b->shape = (!hlld && b->ntiles[4] >= b->cu_count) ? 4 : (hlld ? 1 : 0);
     if (rc == CHIMERA_OK) rc = batch_set_grid(b);
     if (rc != CHIMERA_OK) return fail(rc);
     hipLaunchKernelGGL(batch_init_kernel, dim3((members + 127) / 128), dim3(128), 0, b->stream, b->st,
